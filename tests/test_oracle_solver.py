@@ -1,0 +1,192 @@
+"""Pins the oracle's coarse solve, V-cycle, MGCG, sensitivity and OC
+(stencil_solvers.c, stencil_assembly.c:366-503, stencil_methods.c:808-862,
+gpu_definitions.c:94-157) with scipy/numpy models and invariants."""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+import fem_ref
+import pyoracle
+
+NELX, NELY, NELZ, NL = 16, 8, 8, 3
+
+
+@pytest.fixture(scope="module")
+def setup():
+    g = pyoracle.Grid(NELX, NELY, NELZ, NL)
+    rng = np.random.default_rng(4)
+    x = fem_ref.interior_density(NELX, NELY, NELZ,
+                                 rng.uniform(0.1, 1.0, NELX * NELY * NELZ))
+    assert g.setup(x) == 0
+    yield g, x
+    g.close()
+
+
+def level_ops(x, l):
+    lay = fem_ref.Layout(NELX, NELY, NELZ, l)
+    K = fem_ref.assemble(NELX, NELY, NELZ, l, x)
+    return lay, K
+
+
+def test_coarse_solve_matches_dense(setup):
+    g, x = setup
+    l = NL - 1
+    lay, K = level_ops(x, l)
+    inter = lay.interior_dofs()
+    A = K[inter][:, inter].toarray()
+    pos = {d: t for t, d in enumerate(inter)}
+    fx = [pos[f] for f in lay.fixed_dofs()]
+    A[fx, :] = 0
+    A[:, fx] = 0
+    A[fx, fx] = 1
+    rhs = np.zeros(lay.ndof)
+    rhs[inter] = np.random.default_rng(1).standard_normal(inter.size)
+    got = g.coarse_solve(rhs)
+    ref = np.linalg.solve(A, rhs[inter])
+    assert np.abs(got[inter] - ref).max() <= 1e-11 * np.abs(ref).max()
+
+
+def test_vcycle_is_composition_of_pinned_operators(setup):
+    # numpy V(4,4) with scipy operators: l=0,1 identity-row BCs, l=2 direct
+    g, x = setup
+    lays, Kbc, invD = [], [], []
+    for l in range(NL):
+        lay, K = level_ops(x, l)
+        lays.append(lay)
+        Kb = fem_ref.bc_identity_rows(K, lay.fixed_dofs())
+        Kbc.append(Kb)
+        d = K.diagonal().copy()
+        d[lay.fixed_dofs()] = 1.0
+        inv = np.zeros(lay.ndof)
+        inter = lay.interior_dofs()
+        inv[inter] = 1.0 / d[inter]
+        invD.append(inv)
+    P = [fem_ref.prolongation(NELX, NELY, NELZ, l) for l in range(NL - 1)]
+
+    def smooth(l, z, r):
+        inter = lays[l].interior_dofs()
+        for _ in range(4):
+            t = Kbc[l] @ z
+            z[inter] += 0.6 * invD[l][inter] * (r[inter] - t[inter])
+        return z
+
+    lc = lays[NL - 1]
+    inter_c = lc.interior_dofs()
+    _, Kc = level_ops(x, NL - 1)
+    A = Kc[inter_c][:, inter_c].toarray()
+    pos = {d: t for t, d in enumerate(inter_c)}
+    fx = [pos[f] for f in lc.fixed_dofs()]
+    A[fx, :] = 0
+    A[:, fx] = 0
+    A[fx, fx] = 1
+
+    def vcycle(l, r):
+        z = smooth(l, np.zeros(lays[l].ndof), r)
+        d = r - Kbc[l] @ z
+        rc = P[l].T @ d
+        if l + 2 == NL:
+            zc = np.zeros(lc.ndof)
+            zc[inter_c] = np.linalg.solve(A, rc[inter_c])
+        else:
+            zc = vcycle(l + 1, rc)
+        z = z + P[l] @ zc
+        return smooth(l, z, r)
+
+    r = np.zeros(lays[0].ndof)
+    inter = lays[0].interior_dofs()
+    r[inter] = np.random.default_rng(2).standard_normal(inter.size)
+    r[lays[0].fixed_dofs()] = 0.0
+    ref = vcycle(0, r)
+    got = g.vcycle(x, r)
+    assert np.abs(got - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("case", ["uniform", "random_symbc"])
+def test_mgcg_converges_to_direct_solution(case):
+    # the reference's identity-row BCs converge from uniform density; on
+    # strongly heterogeneous densities only the symmetric-BC mode does
+    g = pyoracle.Grid(NELX, NELY, NELZ, NL, bcsym=(case != "uniform"))
+    rng = np.random.default_rng(5)
+    vals = (np.full(NELX * NELY * NELZ, np.float32(0.2)) if case == "uniform"
+            else rng.uniform(0.1, 1.0, NELX * NELY * NELZ))
+    x = fem_ref.interior_density(NELX, NELY, NELZ, vals)
+    F = g.load()
+    u = np.zeros(g.ndof(0))
+    it, res = g.solve(x, F, u, tol=1e-10)
+    assert res < 1e-10
+    lay, K = level_ops(x, 0)
+    fixed = lay.fixed_dofs()
+    Kbc = fem_ref.bc_identity_rows(K, fixed)
+    assert np.linalg.norm(F - Kbc @ u) <= 1.1e-10 * np.linalg.norm(F)
+    inter = np.setdiff1d(lay.interior_dofs(), fixed)
+    uref = spla.spsolve(K[inter][:, inter].tocsc(), F[inter])
+    assert np.abs(u[inter] - uref).max() <= 1e-8 * np.abs(uref).max()
+    # warm start: a second solve from the converged u stops at iteration 0
+    it2, res2 = g.solve(x, F, u, tol=1e-5)
+    assert it2 == 0 and res2 < 1e-5
+    g.close()
+
+
+def test_compliance_and_sensitivity():
+    g = pyoracle.Grid(NELX, NELY, NELZ, NL)
+    rng = np.random.default_rng(6)
+    vals = rng.uniform(0.1, 1.0, NELX * NELY * NELZ)
+    x = fem_ref.interior_density(NELX, NELY, NELZ, vals)
+    F = g.load()
+    lay, K = level_ops(x, 0)
+    fixed = lay.fixed_dofs()
+    inter = np.setdiff1d(lay.interior_dofs(), fixed)
+
+    def direct(xx):
+        KK = fem_ref.assemble(NELX, NELY, NELZ, 0, xx)
+        uu = np.zeros(lay.ndof)
+        uu[inter] = spla.spsolve(KK[inter][:, inter].tocsc(), F[inter])
+        return uu, KK
+
+    u, K = direct(x)
+    c, dc = g.sensitivity(x, u)
+    assert c == pytest.approx(u @ (K @ u), rel=1e-10)
+    assert c == pytest.approx(F @ u, rel=1e-10)
+    lay0 = fem_ref.Layout(NELX, NELY, NELZ, 0)
+    # h = 1e-4: the O(h^2) truncation is ~1e-8 relative, while at h = 1e-6
+    # the direct-solve rounding (c * 1e-13 / h) reaches 1e-5 relative
+    h = 1e-4
+    for (i, j, k) in [(1, 1, 1), (16, 4, 1), (8, 8, 8), (12, 3, 5)]:
+        e = fem_ref.fine_elem_index(lay0, i, j, k)
+        xp, xm = x.copy(), x.copy()
+        xp[e] += h
+        xm[e] -= h
+        up, Kp = direct(xp)
+        um, Km = direct(xm)
+        fd = (F @ up - F @ um) / (2 * h)
+        assert dc[e] == pytest.approx(fd, rel=1e-6, abs=1e-9 * abs(c))
+    g.close()
+
+
+def test_oc_bisection_invariants():
+    g = pyoracle.Grid(8, 4, 4, 3)
+    rng = np.random.default_rng(7)
+    n = g.nelem()
+    x = rng.uniform(0.05, 0.95, n)
+    dc = -rng.uniform(0.1, 2.0, n)
+    dv = rng.uniform(0.5, 1.5, n)
+    gval = 0.01
+    xnew = g.oc(x, dv, dc, gval)
+    assert (xnew >= 0).all() and (xnew <= 1).all()
+    assert (np.abs(xnew - x) <= 0.2 + 1e-12).all()
+    # the multiplier satisfies g + sum(dv*(xnew-x)) ~ 0 (bisection to 1e-6)
+    gt = gval + np.sum(dv * (xnew - x))
+    assert abs(gt) < 1e-3 * np.sum(dv)
+    g.close()
+
+
+def test_reference_bc_rows_stagnate_on_random_density():
+    # documents the reference behaviour the symmetric-BC mode exists for
+    g = pyoracle.Grid(NELX, NELY, NELZ, NL)
+    rng = np.random.default_rng(5)
+    x = fem_ref.interior_density(NELX, NELY, NELZ,
+                                 rng.uniform(0.1, 1.0, NELX * NELY * NELZ))
+    u = np.zeros(g.ndof(0))
+    it, res = g.solve(x, g.load(), u, tol=1e-5, maxiter=200)
+    assert it == 199 and res > 1e-3
+    g.close()
