@@ -1,0 +1,325 @@
+/*
+ * top3d_hip.h -- C ABI of the MI355X (gfx950) MGCG state-solve library.
+ *
+ * Drop-in replacement for the device path of topopt/TopOpt-in-OpenMP-GPU:
+ * the reference's structs and entry points keep their names, field order and
+ * argument meaning; each declaration cites the reference declaration it
+ * replaces.  Element types are the reference's typedef names fixed to the
+ * FP64 configuration (definitions.h:25-29 re-typed): every array that crosses
+ * this boundary is double.  The mixed-precision solve keeps FP32 copies
+ * inside the library (see top3d_set_precision).
+ *
+ * Memory: the reference hands host pointers to OpenMP target regions and
+ * relies on the present table filled by `#pragma omp target enter data`.
+ * This library keeps its own present table: enter_data() / top3d_map()
+ * create device mirrors of host arrays, top3d_update_to/from() replace
+ * `#pragma omp target update`, and every entry point accepts either a mapped
+ * host pointer (its mirror is used) or a device pointer.  Unmapped host
+ * pointers are rejected (top3d_last_error() reports it).
+ *
+ * Errors: entry points keep the reference's void signatures; failures are
+ * recorded in a thread-local status read with top3d_last_error().
+ * All device work is ordered on one HIP stream (top3d_set_stream).
+ */
+#ifndef TOP3D_HIP_H
+#define TOP3D_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* definitions.h:25-29, FP64 configuration */
+typedef double MTYPE;
+typedef double STYPE;
+typedef double CTYPE;
+typedef double DTYPE;
+
+/* definitions.h:17,23 */
+#define STENCIL_SIZE_Y 128
+#define number_of_matrix_free_levels 2
+
+/* definitions.h:31-34 */
+struct FixedDofs {
+  uint_fast32_t n;
+  uint_fast32_t *idx;
+};
+
+/* definitions.h:36-53 */
+struct gridContext {
+  double E0;
+  double Emin;
+  double nu;
+  double elementSizeX;
+  double elementSizeY;
+  double elementSizeZ;
+  uint_fast32_t nelx;
+  uint_fast32_t nely;
+  uint_fast32_t nelz;
+  uint_fast32_t wrapx;
+  uint_fast32_t wrapy;
+  uint_fast32_t wrapz;
+  double penal;
+  MTYPE **precomputedKE;
+  struct FixedDofs *fixedDofs;
+};
+
+/* stencil_assembly.h:8-15 */
+struct CSRMatrix {
+  uint64_t nnz;
+  int32_t nrows;
+  int *rowOffsets;
+  int *colIndex;
+  MTYPE *vals;
+};
+
+/* stencil_assembly.h:17-27.  The CHOLMOD handles of the reference become
+ * opaque slots; `cholmodCommon` holds this library's coarse-solver object.  */
+struct CoarseSolverData {
+  void *cholmodCommon;
+  void *sparseMatrix;
+  void *factoredMatrix;
+  void *rhs;
+  void *solution;
+  void *Y_workspace;
+  void *E_workspace;
+};
+
+/* gpu_definitions.h:7-23.  r,p,q,z and the per-level arrays hold DEVICE
+ * pointers owned by the library (allocateSolverData).                        */
+struct SolverDataBuffer {
+  CTYPE *r;
+  CTYPE *p;
+  CTYPE *q;
+  CTYPE *z;
+  MTYPE **invD;
+  CTYPE **dmg;
+  CTYPE **rmg;
+  CTYPE **zmg;
+  struct CSRMatrix *coarseMatrices;
+  struct CoarseSolverData bottomSolver;
+};
+
+/* gpu_definitions.h:25-37 */
+struct gpuNode {
+  int id;
+  struct gpuNode *prev;
+  struct gpuNode *next;
+  struct gridContext *gc;
+  int offset_x;
+  int x_global;
+};
+
+struct gpuGrid {
+  int num_targets;
+  struct gpuNode *target;
+};
+
+/* ---- grid set-up (stencil_grid_utility.h:5-17, gpu_definitions.h:39-56) -- */
+void setupGC(struct gridContext *gc, const int nl, const int nelx,
+             const int nely, const int nelz);
+void freeGC(struct gridContext *gc, const int nl);
+void setFixedDof_halo(struct gridContext *gc, const int l);
+void allocateZeroPaddedStateField(const struct gridContext gc, const int l,
+                                  CTYPE **v);
+void allocateZeroPaddedStateField_MTYPE(const struct gridContext gc,
+                                        const int l, MTYPE **v);
+void allocateZeroPaddedStateField_STYPE(const struct gridContext gc,
+                                        const int l, STYPE **v);
+void computePadding(const struct gridContext *gc, const int l, int *ncell,
+                    int *wrapx, int *wrapy, int *wrapz, uint_fast32_t *ndof);
+void gridContextToGPUGrid(struct gridContext *gc, struct gpuGrid *gpu_gc,
+                          const int nl, const int verbose);
+void freeGPUGrid(struct gpuGrid *gpu_gc, const int nl);
+/* local_matrix.h:19-21 */
+void getKEsubspace(MTYPE *KEarray, const MTYPE nu, const int l);
+
+/* ---- matrix-free operators (stencil_methods.h:6-28) ---------------------- */
+void applyStateOperator_stencil(const struct gpuNode *gpu_node,
+                                const DTYPE *x, const CTYPE *in, CTYPE *out);
+void applyStateOperatorSubspace_halo(const struct gpuNode *gpu_node,
+                                     const int l, const DTYPE *x, CTYPE *in,
+                                     CTYPE *out);
+void getComplianceAndSensetivity_halo(const struct gpuNode *gpu_node,
+                                      const DTYPE *x, STYPE *u, DTYPE *c,
+                                      DTYPE *dcdx);
+void projectToFinerGrid_halo(const struct gpuNode *gpu_node, const int l,
+                             const CTYPE *ucoarse, CTYPE *ufine);
+void projectToCoarserGrid_halo(const struct gpuNode *gpu_node, const int l,
+                               const CTYPE *ufine, CTYPE *ucoarse);
+void assembleInvertedMatrixDiagonalSubspace_halo(
+    const struct gpuNode *gpu_node, const DTYPE *x, const int l, MTYPE *diag);
+
+/* ---- smoothers and solver (stencil_solvers.h:8-32) ----------------------- */
+void smoothDampedJacobi_halo(const struct gpuNode *gpu_node, const DTYPE *x,
+                             const uint_fast32_t nswp, const CTYPE omega,
+                             const MTYPE *invD, CTYPE *u, const CTYPE *b,
+                             CTYPE *tmp);
+void smoothDampedJacobiSubspace_halo(const struct gpuNode *gpu_node,
+                                     const DTYPE *x, const int l,
+                                     const uint_fast32_t nswp,
+                                     const CTYPE omega, const MTYPE *invD,
+                                     CTYPE *u, const CTYPE *b, CTYPE *tmp);
+void smoothDampedJacobiSubspaceMatrix_halo(
+    const struct gridContext *gc, const struct CSRMatrix M, const int l,
+    const uint_fast32_t nswp, const CTYPE omega, const MTYPE *invD, CTYPE *u,
+    const CTYPE *b, CTYPE *tmp);
+void solveStateMG_halo(const struct gpuGrid *gpu_gc, DTYPE *x, const int nswp,
+                       const int nl, const CTYPE tol,
+                       struct SolverDataBuffer *data, int *finalIter,
+                       float *finalRes, CTYPE *b, STYPE *u);
+void allocateSolverData(const struct gridContext gc, const int nl,
+                        struct SolverDataBuffer *data);
+void freeSolverData(struct SolverDataBuffer *data, const int nl);
+
+/* ---- assembled coarse levels + direct solve (stencil_assembly.h:29-55) ---
+ * CSR arrays are host arrays in the reference's layout; assembly, SpMV,
+ * factorization and solve run on the GPU.                                    */
+void allocateSubspaceMatrix(const struct gridContext gc, const int l,
+                            struct CSRMatrix *M);
+void freeSubspaceMatrix(struct CSRMatrix *M);
+void assembleSubspaceMatrix(const struct gridContext gc, const int l,
+                            const DTYPE *x, struct CSRMatrix M, MTYPE *tmp);
+void applyStateOperatorSubspaceMatrix(const struct gridContext gc, const int l,
+                                      const struct CSRMatrix M,
+                                      const CTYPE *in, CTYPE *out);
+void initializeCholmod(const struct gridContext gc, const int l,
+                       struct CoarseSolverData *solverData,
+                       const struct CSRMatrix M);
+void finishCholmod(const struct gridContext gc, const int l,
+                   struct CoarseSolverData *solverData,
+                   const struct CSRMatrix M, const int verbose);
+void factorizeSubspaceMatrix(const struct gridContext gc, const int l,
+                             struct CoarseSolverData solverData,
+                             const struct CSRMatrix M);
+void solveSubspaceMatrix(const struct gridContext gc, const int l,
+                         struct CoarseSolverData solverData, const CTYPE *in,
+                         CTYPE *out);
+
+/* ---- design loop kernels (gpu_definitions.h:58-86,
+ *      stencil_optimization.h:5-15) ---------------------------------------- */
+DTYPE compute_volume(const struct gridContext gc, DTYPE *xPhys);
+float compute_change(DTYPE *x, DTYPE *xnew, const int nelem);
+void update_solution(const DTYPE *x, DTYPE *xnew, const DTYPE *dv,
+                     const DTYPE *dc, const int nelem, const DTYPE g);
+void enter_data(const struct gridContext gridContext,
+                const struct SolverDataBuffer solverData,
+                const struct gpuGrid gpu_gc, const DTYPE *xPhys,
+                const DTYPE *x, const DTYPE *xnew, const STYPE *U,
+                const CTYPE *F, const DTYPE *dc, const int nelem,
+                const int nl);
+void applyDensityFilter(const struct gridContext gc, const DTYPE rmin,
+                        const DTYPE *rho, DTYPE *out);
+void applyDensityFilterGradient(const struct gridContext gc, const DTYPE rmin,
+                                DTYPE *v);
+void top3dmgcg(const uint_fast32_t nelx, const uint_fast32_t nely,
+               const uint_fast32_t nelz, const DTYPE volfrac,
+               const DTYPE rmin, const uint_fast32_t nl, const float cgtol,
+               const uint_fast32_t cgmax, const int verbose,
+               const int write_result, const int max_iterations);
+/* stencil_optimization.h:17-22; densityArray is a host array in this
+ * library's padded element layout */
+void writeDensityVtkFile(const int nelx, const int nely, const int nelz,
+                         const DTYPE *densityArray, const char *filename);
+void writeDensityVtkFileWithHalo(const int nelx, const int nely,
+                                 const int nelz, const DTYPE *densityArray,
+                                 const char *filename);
+
+/* ======================================================================== */
+/* Entry points with no counterpart in the reference.                        */
+
+/* status codes of top3d_last_error() */
+#define TOP3D_OK 0
+#define TOP3D_ERR_HIP 1          /* a HIP call failed */
+#define TOP3D_ERR_ARG 2          /* invalid argument / unsupported layout */
+#define TOP3D_ERR_UNMAPPED 3     /* host pointer without device mirror */
+#define TOP3D_ERR_NOT_SPD 4      /* coarse Cholesky met a non-positive pivot */
+#define TOP3D_ERR_NO_CONVERGE 5  /* CG stopped at maxIter above tol */
+#define TOP3D_ERR_NONFINITE 6    /* NaN/Inf in the CG recurrence */
+int top3d_last_error(void);
+const char *top3d_last_error_string(void);
+void top3d_clear_error(void);
+
+/* precision of solveStateMG_halo for a solver buffer:
+ * TOP3D_FP64  every vector, operator and reduction in double (plain form);
+ * TOP3D_MIXED V-cycle (level operators, smoothers, transfers, level vectors,
+ *             densities) in FP32; outer CG, its matvec, dot products and the
+ *             coarse Cholesky in FP64.                                       */
+#define TOP3D_FP64 0
+#define TOP3D_MIXED 1
+void top3d_set_precision(struct SolverDataBuffer *data, int mode);
+/* boundary-condition convention of the matrix-free levels:
+ * TOP3D_BC_REFERENCE  fixed rows replaced by identity, columns kept
+ *                     (stencil_methods.c:424-427, 515-518);
+ * TOP3D_BC_SYMMETRIC  columns of fixed DOFs dropped as well (symmetric
+ *                     V-cycle; converges on heterogeneous densities).       */
+#define TOP3D_BC_REFERENCE 0
+#define TOP3D_BC_SYMMETRIC 1
+void top3d_set_bc_mode(struct SolverDataBuffer *data, int mode);
+void top3d_set_max_iter(struct SolverDataBuffer *data, int max_iter);
+/* modes top3dmgcg() applies to the solver buffer it allocates */
+void top3d_set_default_precision(int mode);
+void top3d_set_default_bc_mode(int mode);
+/* the cantilever load of top3dmgcg (stencil_optimization.c:39-95), added
+ * into F (host array in the level-0 halo layout) */
+void top3d_cantilever_load(const struct gridContext *gc, CTYPE *F);
+
+/* present table (replaces `#pragma omp target enter/exit data/update`) */
+int top3d_map(const void *host, uint64_t bytes);          /* alloc + copy in */
+int top3d_map_alloc(const void *host, uint64_t bytes);    /* alloc, no copy */
+int top3d_unmap(const void *host);
+int top3d_update_to(const void *host, uint64_t bytes);
+int top3d_update_from(void *host, uint64_t bytes);
+void *top3d_device_ptr(const void *host_or_device);
+
+/* stream on which all device work is ordered (0 = the library's own) */
+void top3d_set_stream(void *hip_stream);
+void *top3d_get_stream(void);
+int top3d_synchronize(void);
+
+/* Statistics of the last solveStateMG_halo on a buffer. */
+struct Top3dSolveStats {
+  int iterations;        /* CG iterations run (finalIter + 1) */
+  double relres;         /* final relative residual (FP64) */
+  double setup_ms;       /* per-solve set-up: levels, diagonals, factor */
+  double solve_ms;       /* CG loop */
+  int status;            /* TOP3D_OK or an error code */
+};
+void top3d_solver_stats(const struct SolverDataBuffer *data,
+                        struct Top3dSolveStats *out);
+
+/* Compliance c(x) = sum_e E(x_e) u_e^T KE u_e with u from the solve and its
+ * gradient with respect to the filtered densities, written for callers that
+ * hold device arrays (the torch.autograd wrapper):
+ *   forward : xPhys (device, padded element array) -> u (device), c (host)
+ *   backward: dc/dxPhys_e = -3 (E0-Emin) xPhys_e^2 u_e^T KE u_e, scaled by gc
+ *             and pulled back through the density filter: grad = H^T (dc/Hs).
+ * rmin <= 0 skips the filter.                                                */
+int top3d_compliance_forward(const struct gpuGrid *gpu_gc,
+                             struct SolverDataBuffer *data, int nl,
+                             double tol, const double *xPhys_dev,
+                             const double *F_dev, double *U_dev, double *c);
+int top3d_compliance_backward(const struct gpuGrid *gpu_gc, double rmin,
+                              const double *xPhys_dev, const double *U_dev,
+                              double grad_c, double *grad_x_dev);
+
+/* The coarsest-level dense solver on its own: A (device, n x n row-major,
+ * symmetric positive definite, nonzero only within |i-j| <= bw) is replaced
+ * by A^{-1} (banded Cholesky on FP64 MFMA, then banded solves).  Returns
+ * TOP3D_OK or TOP3D_ERR_NOT_SPD.                                            */
+int top3d_dense_spd_inverse(double *A_dev, int n, int bw);
+
+/* FP32 variants of the level operators, the arithmetic of the mixed
+ * V-cycle, exposed for parity tests (device float arrays, same layout).    */
+void applyStateOperator_stencil_f32(const struct gpuNode *gpu_node,
+                                    const DTYPE *x, const float *in,
+                                    float *out);
+void applyStateOperatorSubspace_halo_f32(const struct gpuNode *gpu_node,
+                                         const int l, const DTYPE *x,
+                                         const float *in, float *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TOP3D_HIP_H */

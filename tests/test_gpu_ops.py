@@ -1,0 +1,191 @@
+"""GPU parity of the level operators, transfers, diagonals, smoothers,
+assembled coarse levels and design-loop kernels against the CPU oracle.
+All calls go through the C ABI (libtop3d_hip.so via ctypes).  FP64 results
+must agree to 1e-13 normwise (1e-12 where assembly order differs); FP32
+kernels of the mixed V-cycle to FP32 rounding (1e-5)."""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+import fem_ref
+import pyoracle
+import top3d_amd
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(16, 8, 8, 3), (24, 16, 8, 4)]
+
+
+def relerr(got, ref):
+    return np.abs(np.asarray(got) - np.asarray(ref)).max() / max(np.abs(ref).max(), 1e-300)
+
+
+@pytest.fixture(scope="module", params=SIZES, ids=lambda s: "x".join(map(str, s)))
+def pair(request):
+    nelx, nely, nelz, nl = request.param
+    print(f"[gpu ops] grid {nelx}x{nely}x{nelz} nl={nl}", flush=True)
+    g = top3d_amd.Grid(nelx, nely, nelz, nl)
+    o = pyoracle.Grid(nelx, nely, nelz, nl)
+    assert g.padding(0) == o.padding(0)
+    rng = np.random.default_rng(nelx)
+    vals = rng.uniform(0.1, 1.0, nelx * nely * nelz)
+    xg = g.density(vals)
+    xo = xg.cpu().numpy()
+    yield g, o, xg, xo, rng
+    g.close()
+    o.close()
+
+
+def cuda(a, dtype=torch.float64):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0", dtype)
+
+
+def test_fine_operator_fp64(pair):
+    g, o, xg, xo, rng = pair
+    v = rng.standard_normal(g.ndof(0))
+    got = g.apply(xg, cuda(v)).cpu().numpy()
+    ref = o.matvec0(xo, v)
+    assert relerr(got, ref) <= 1e-13
+
+
+def test_fine_operator_fp32(pair):
+    g, o, xg, xo, rng = pair
+    v = rng.standard_normal(g.ndof(0))
+    got = g.apply_f32(xg, cuda(v, torch.float32)).cpu().numpy()
+    ref = o.matvec0(xo, v.astype(np.float32).astype(np.float64))
+    assert relerr(got, ref) <= 2e-6
+
+
+def test_level1_operator(pair):
+    g, o, xg, xo, rng = pair
+    v = rng.standard_normal(g.ndof(1))
+    got = g.apply_level(1, xg, cuda(v)).cpu().numpy()
+    ref = o.matvec_l(1, xo, v)
+    assert relerr(got, ref) <= 1e-13
+    got32 = g.apply_level_f32(1, xg, cuda(v, torch.float32)).cpu().numpy()
+    assert relerr(got32, o.matvec_l(1, xo, v.astype(np.float32).astype(np.float64))) <= 2e-6
+
+
+def test_level2_operator_identity_rows(pair):
+    g, o, xg, xo, rng = pair
+    v = rng.standard_normal(g.ndof(2))
+    got = g.apply_level(2, xg, cuda(v)).cpu().numpy()
+    ref = o.matvec_l(2, xo, v)
+    assert relerr(got, ref) <= 1e-12
+
+
+def test_transfers(pair):
+    g, o, xg, xo, rng = pair
+    for l in range(g.nl - 1):
+        vc = rng.standard_normal(g.ndof(l + 1))
+        assert relerr(g.prolong(l, cuda(vc)).cpu().numpy(), o.prolong(l, vc)) <= 1e-15
+        vf = rng.standard_normal(g.ndof(l))
+        assert relerr(g.restrict(l, cuda(vf)).cpu().numpy(), o.restrict(l, vf)) <= 1e-15
+
+
+def test_inverse_diagonals(pair):
+    g, o, xg, xo, rng = pair
+    for l in range(g.nl):
+        assert relerr(g.invdiag(xg, l).cpu().numpy(), o.invdiag(xo, l)) <= 1e-13
+
+
+def test_jacobi_smoothers(pair):
+    g, o, xg, xo, rng = pair
+    for l in (0, 1):
+        inv = o.invdiag(xo, l)
+        u = rng.standard_normal(g.ndof(l))
+        b = rng.standard_normal(g.ndof(l))
+        ug = g.jacobi(xg, 3, cuda(inv), cuda(u), cuda(b), l=l).cpu().numpy()
+        uo = u.copy()
+        tmp = np.zeros_like(u)
+        P = ctypes.c_void_p
+        ptrs = [P(inv.ctypes.data), P(uo.ctypes.data), P(b.ctypes.data), P(tmp.ctypes.data)]
+        if l == 0:
+            fn = o.lib.or_jacobi0
+            fn.argtypes = [P, P, ctypes.c_int, ctypes.c_double, P, P, P, P]
+            fn(P(o.h), P(xo.ctypes.data), 3, 0.6, *ptrs)
+        else:
+            fn = o.lib.or_jacobi_l
+            fn.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double, P, P, P, P]
+            fn(P(o.h), P(xo.ctypes.data), l, 3, 0.6, *ptrs)
+        assert relerr(ug, uo) <= 1e-12
+
+
+def test_assembled_level_csr(pair):
+    g, o, xg, xo, rng = pair
+    lib = g.lib
+    for l in range(2, g.nl):
+        M = top3d_amd.CSRMatrix()
+        lib.allocateSubspaceMatrix(g.gc, l, ctypes.byref(M))
+        tmp = np.zeros(M.nrows)
+        lib.assembleSubspaceMatrix(g.gc, l, top3d_amd._ptr(xg), M,
+                                   tmp.ctypes.data_as(ctypes.c_void_p))
+        top3d_amd.check("assembleSubspaceMatrix")
+        vals = np.ctypeslib.as_array(M.vals, (M.nnz,)).copy()
+        ref = o.csr(l, xo)
+        assert relerr(vals, ref.data) <= 1e-12
+        # CSR SpMV in the halo layout
+        v = np.zeros(g.ndof(l))
+        lay = fem_ref.Layout(g.nelx, g.nely, g.nelz, l)
+        inter = lay.interior_dofs()
+        v[inter] = rng.standard_normal(inter.size)
+        out = g.zeros(l)
+        lib.applyStateOperatorSubspaceMatrix(g.gc, l, M, top3d_amd._ptr(cuda(v)),
+                                             top3d_amd._ptr(out))
+        top3d_amd.check("applyStateOperatorSubspaceMatrix")
+        refo = np.zeros(g.ndof(l))
+        refo[inter] = ref @ v[inter]
+        assert relerr(out.cpu().numpy(), refo) <= 1e-12
+        lib.freeSubspaceMatrix(ctypes.byref(M))
+
+
+def test_coarse_direct_solve(pair):
+    g, o, xg, xo, rng = pair
+    lib = g.lib
+    l = g.nl - 1
+    M = top3d_amd.CSRMatrix()
+    lib.allocateSubspaceMatrix(g.gc, l, ctypes.byref(M))
+    lib.assembleSubspaceMatrix(g.gc, l, top3d_amd._ptr(xg), M, None)
+    cs = top3d_amd.CoarseSolverData()
+    lib.initializeCholmod(g.gc, l, ctypes.byref(cs), M)
+    lib.factorizeSubspaceMatrix(g.gc, l, cs, M)
+    top3d_amd.check("factorizeSubspaceMatrix")
+    lay = fem_ref.Layout(g.nelx, g.nely, g.nelz, l)
+    rhs = np.zeros(g.ndof(l))
+    rhs[lay.interior_dofs()] = rng.standard_normal(lay.interior_dofs().size)
+    out = g.zeros(l)
+    lib.solveSubspaceMatrix(g.gc, l, cs, top3d_amd._ptr(cuda(rhs)), top3d_amd._ptr(out))
+    top3d_amd.check("solveSubspaceMatrix")
+    assert o.setup(xo) == 0
+    ref = o.coarse_solve(rhs)
+    assert relerr(out.cpu().numpy(), ref) <= 1e-10
+    lib.finishCholmod(g.gc, l, ctypes.byref(cs), M, 0)
+    lib.freeSubspaceMatrix(ctypes.byref(M))
+
+
+def test_sensitivity_filter_volume_change_oc(pair):
+    g, o, xg, xo, rng = pair
+    u = rng.standard_normal(g.ndof(0))
+    c, dc = g.sensitivity(xg, cuda(u))
+    co, dco = o.sensitivity(xo, u)
+    inter = g.interior_elements()
+    assert abs(c - co) <= 1e-12 * abs(co)
+    assert relerr(dc.cpu().numpy()[inter], dco[inter]) <= 1e-12
+    rmin = 1.5
+    assert relerr(g.filter(rmin, xg).cpu().numpy()[inter], o.filter(rmin, xo)[inter]) <= 1e-14
+    v = g.density(rng.uniform(-1, 1, inter.size))
+    assert relerr(g.filter_grad(rmin, v).cpu().numpy()[inter],
+                  o.filter_grad(rmin, v.cpu().numpy())[inter]) <= 1e-14
+    assert g.volume(xg) == pytest.approx(o.volume(xo), rel=1e-13)
+    dv = g.filter_grad(rmin, torch.ones_like(xg))
+    dcg = g.filter_grad(rmin, dc)
+    xnew = torch.zeros_like(xg)
+    g.oc(xg, xnew, dv, dcg, 0.01)
+    ref_new = o.oc(xo, dv.cpu().numpy(), dcg.cpu().numpy(), 0.01)
+    assert np.abs(xnew.cpu().numpy() - ref_new).max() <= 1e-12
+    xc = xg.clone()
+    ch = g.change(xc, xnew)
+    assert ch == pytest.approx(float(np.float32(np.abs(xo - ref_new).max())), rel=1e-6)
+    assert torch.equal(xc, xnew)

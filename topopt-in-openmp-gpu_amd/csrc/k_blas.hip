@@ -1,0 +1,176 @@
+// Vector kernels of the CG recurrence: dot products and norms as wavefront
+// reductions with a fixed-partition second stage (deterministic), the fused
+// CG updates with scalars kept on the device, conversions and fills.
+#include "t3d_internal.hpp"
+
+namespace t3d {
+
+constexpr int kRedBlock = 256;
+constexpr int kRedGrid = 1024;  // partial sums per reduction
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ double block_sum(double v) {
+  __shared__ double sm[kRedBlock / 64];
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sm[w] = v;
+  __syncthreads();
+  double s = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < kRedBlock / 64; i++) s += sm[i];
+  return s;  // valid in thread 0
+}
+
+// stage 1: partials[block] = sum over a grid-stride range of a[i]*b[i]
+template <typename T>
+__global__ __launch_bounds__(kRedBlock) void k_dot_partial(const T *__restrict__ a,
+                                                           const T *__restrict__ b,
+                                                           int64_t n,
+                                                           double *__restrict__ partials) {
+  double s = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    s += (double)a[i] * (double)b[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// stage 2: one block sums the partials in a fixed order
+__global__ __launch_bounds__(kRedBlock) void k_sum_partials(const double *__restrict__ partials,
+                                                            int n, double *__restrict__ out) {
+  double s = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += partials[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) *out = s;
+}
+
+template <typename T>
+void launch_dot(const T *a, const T *b, int64_t n, double *partials, double *result) {
+  hipLaunchKernelGGL((k_dot_partial<T>), dim3(kRedGrid), dim3(kRedBlock), 0, stream(), a,
+                     b, n, partials);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kRedBlock), 0, stream(), partials,
+                     kRedGrid, result);
+}
+
+// Device-resident CG scalars (layout shared with the solver):
+//   s[0] rho, s[1] rho_old, s[2] p.q, s[3] r.r, s[4] b.b
+// p = z (first) or p = (rho/rho_old) p + z  (stencil_solvers.c:350-362)
+__global__ void k_pupdate(int64_t n, const double *__restrict__ s, double *__restrict__ p,
+                          const double *__restrict__ z, int first) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (first) {
+    p[i] = z[i];
+  } else {
+    const double beta = s[0] / s[1];
+    p[i] = beta * p[i] + z[i];
+  }
+}
+
+void launch_pupdate(int64_t n, const double *scal, double *p, const double *z, int first) {
+  const int bs = 256;
+  hipLaunchKernelGGL(k_pupdate, dim3((n + bs - 1) / bs), dim3(bs), 0, stream(), n, scal, p,
+                     z, first);
+}
+
+// u += alpha p; r -= alpha q with alpha = rho / p.q, and rho_old = rho
+// (stencil_solvers.c:366-377); only s[0] and s[2] are read here, so thread 0
+// may store s[1].
+__global__ void k_axpy_cg(int64_t n, double *__restrict__ s, double *__restrict__ u,
+                          double *__restrict__ r, const double *__restrict__ p,
+                          const double *__restrict__ q) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double rho = s[0];
+  const double alpha = rho / s[2];
+  u[i] += alpha * p[i];
+  r[i] -= alpha * q[i];
+  if (i == 0) s[1] = rho;
+}
+
+void launch_axpy_cg(int64_t n, double *scal, double *u, double *r, const double *p,
+                    const double *q) {
+  const int bs = 256;
+  hipLaunchKernelGGL(k_axpy_cg, dim3((n + bs - 1) / bs), dim3(bs), 0, stream(), n, scal, u,
+                     r, p, q);
+}
+
+template <typename TO, typename TI>
+__global__ void k_convert(int64_t n, const TI *__restrict__ in, TO *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (TO)in[i];
+}
+template <typename TO, typename TI>
+void launch_convert(int64_t n, const TI *in, TO *out) {
+  const int bs = 256;
+  hipLaunchKernelGGL((k_convert<TO, TI>), dim3((n + bs - 1) / bs), dim3(bs), 0, stream(), n,
+                     in, out);
+}
+
+template <typename T>
+__global__ void k_fill(int64_t n, T *__restrict__ out, T v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = v;
+}
+template <typename T>
+void launch_fill(int64_t n, T *out, T v) {
+  const int bs = 256;
+  hipLaunchKernelGGL((k_fill<T>), dim3((n + bs - 1) / bs), dim3(bs), 0, stream(), n, out, v);
+}
+
+template <typename T>
+__global__ void k_sub(int64_t n, const T *__restrict__ a, const T *__restrict__ b,
+                      T *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a[i] - b[i];
+}
+template <typename T>
+void launch_sub(int64_t n, const T *a, const T *b, T *out) {
+  const int bs = 256;
+  hipLaunchKernelGGL((k_sub<T>), dim3((n + bs - 1) / bs), dim3(bs), 0, stream(), n, a, b,
+                     out);
+}
+
+template <typename T>
+__global__ void k_add(int64_t n, const T *__restrict__ a, T *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] += a[i];
+}
+template <typename T>
+void launch_add(int64_t n, const T *a, T *out) {
+  const int bs = 256;
+  hipLaunchKernelGGL((k_add<T>), dim3((n + bs - 1) / bs), dim3(bs), 0, stream(), n, a, out);
+}
+
+// out[f] = in[f] for the fixed DOFs of level l (BC rows of the operators)
+__global__ void k_copy_fixed(int64_t n, const int64_t *__restrict__ idx,
+                             const double *__restrict__ in, double *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[idx[i]] = in[idx[i]];
+}
+void launch_copy_fixed(const DevGrid &g, int l, const double *in, double *out) {
+  const int64_t n = g.nfixed[l];
+  if (n <= 0) return;
+  const int bs = 256;
+  hipLaunchKernelGGL(k_copy_fixed, dim3((n + bs - 1) / bs), dim3(bs), 0, stream(), n,
+                     g.fixed[l], in, out);
+}
+
+template void launch_dot<double>(const double *, const double *, int64_t, double *, double *);
+template void launch_dot<float>(const float *, const float *, int64_t, double *, double *);
+template void launch_convert<float, double>(int64_t, const double *, float *);
+template void launch_convert<double, float>(int64_t, const float *, double *);
+template void launch_convert<double, double>(int64_t, const double *, double *);
+template void launch_fill<double>(int64_t, double *, double);
+template void launch_fill<float>(int64_t, float *, float);
+template void launch_sub<double>(int64_t, const double *, const double *, double *);
+template void launch_sub<float>(int64_t, const float *, const float *, float *);
+template void launch_add<double>(int64_t, const double *, double *);
+template void launch_add<float>(int64_t, const float *, float *);
+
+}  // namespace t3d

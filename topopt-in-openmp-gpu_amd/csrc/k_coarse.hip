@@ -1,0 +1,239 @@
+// Assembled coarse levels on the GPU (replacing the host CSR assembly of
+// stencil_assembly.c:126-260 and the PCIe round trips of
+// stencil_solvers.c:172-250).
+//
+// Level-l element matrices are built by Galerkin recursion,
+//   Ke_l(E) = sum_{children c} Q_c^T Ke_{l-1}(c) Q_c,
+// with Q_c the trilinear interpolation from the parent's 8 nodes to child
+// c's nodes.  Because the reference's sub-cell matrices at level l use the
+// fine elements' own quadrature points (local_matrix.c:268-293), this equals
+// its sum_f E_f KE_l[f] exactly in exact arithmetic.  Element matrices are
+// then gathered into a 27-point node stencil with symmetric elimination of
+// the fixed DOFs (stencil_assembly.c:226-259), which serves the smoother
+// levels and the dense coarsest matrix.
+#include "t3d_internal.hpp"
+
+namespace t3d {
+
+__host__ __device__ constexpr int cn_x(int n) { return (n == 1 || n == 2 || n == 5 || n == 6) ? 1 : 0; }
+__host__ __device__ constexpr int cn_y(int n) { return (n == 0 || n == 1 || n == 4 || n == 5) ? 1 : 0; }
+__host__ __device__ constexpr int cn_z(int n) { return n >= 4 ? 1 : 0; }
+__host__ __device__ constexpr int cn_of(int ox, int oy, int oz) {
+  return oz * 4 + (oy ? (ox ? 1 : 0) : (ox ? 2 : 3));
+}
+
+// weight of parent node B at child c's node b (trilinear, per direction)
+__device__ __forceinline__ double qweight(int cx, int cy, int cz, int b, int B) {
+  const int p[3] = {cx + cn_x(b), cy + cn_y(b), cz + cn_z(b)};  // in half units 0..2
+  const int q[3] = {cn_x(B), cn_y(B), cn_z(B)};
+  double w = 1.0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double t = 0.5 * p[d];
+    w *= q[d] ? t : 1.0 - t;
+  }
+  return w;
+}
+
+// One workgroup (576 threads, one per matrix entry) per parent element.
+// SRC = 0: children are level-1 elements built from fine moduli and KE_1;
+// SRC = 1: children are stored level-(l-1) element matrices.
+template <int SRC>
+__global__ __launch_bounds__(576) void k_galerkin(
+    int nelx, int nely, int nelz,        // parent level elements
+    int cnely, int cnelz,                // child level elements (SRC = 1)
+    int few, int fez, const double *__restrict__ KE1, const double *__restrict__ E,
+    const double *__restrict__ Kchild, double *__restrict__ Kparent) {
+  __shared__ double Kc[576];
+  __shared__ double Tm[576];
+  const int t = threadIdx.x;
+  const int r = t / 24, s = t % 24;
+  const int64_t pe = blockIdx.x;  // compact parent index ((I*nelz + K)*nely + J), 0-based
+  const int J = (int)(pe % nely);
+  const int K = (int)((pe / nely) % nelz);
+  const int I = (int)(pe / ((int64_t)nely * nelz));
+  double acc = 0.0;
+  for (int c = 0; c < 8; c++) {
+    const int cx = cn_x(c), cy = cn_y(c), cz = cn_z(c);  // child octant
+    const int ci = 2 * I + cx, cj = 2 * J + cy, ck = 2 * K + cz;  // child, 0-based
+    double kv;
+    if (SRC == 0) {
+      kv = 0.0;
+      for (int ii = 0; ii < 2; ii++)
+        for (int kk = 0; kk < 2; kk++)
+          for (int jj = 0; jj < 2; jj++) {
+            const int64_t fe = ((int64_t)(2 * ci + ii + 1) * fez + (2 * ck + kk + 1)) * few +
+                               (2 * cj + jj + 1);
+            kv += E[fe] * KE1[576 * (4 * ii + 2 * kk + jj) + t];
+          }
+    } else {
+      const int64_t ce = ((int64_t)ci * cnelz + ck) * cnely + cj;
+      kv = Kchild[576 * ce + t];
+    }
+    Kc[t] = kv;
+    __syncthreads();
+    // Tm[a][S] = sum_b Kc[a][3b + cs] Q[b][Snode]
+    {
+      const int a = r, S = s, Sn = S / 3, cs = S % 3;
+      double v = 0.0;
+#pragma unroll
+      for (int b = 0; b < 8; b++) v += Kc[24 * a + 3 * b + cs] * qweight(cx, cy, cz, b, Sn);
+      Tm[t] = v;
+    }
+    __syncthreads();
+    // P[R][S] += sum_a Q[a][Rnode] Tm[3a + cr][S]
+    {
+      const int Rn = r / 3, cr = r % 3;
+      double v = 0.0;
+#pragma unroll
+      for (int a = 0; a < 8; a++) v += qweight(cx, cy, cz, a, Rn) * Tm[24 * (3 * a + cr) + s];
+      acc += v;
+    }
+    __syncthreads();
+  }
+  Kparent[576 * pe + t] = acc;
+}
+
+void launch_elem_level2(const DevGrid &g, const double *E, double *Ke2) {
+  const Level &L = g.lv[2];
+  hipLaunchKernelGGL((k_galerkin<0>), dim3(L.elements()), dim3(576), 0, stream(), L.nelx,
+                     L.nely, L.nelz, 0, 0, g.ewy, g.ewz, g.KE[1], E, nullptr, Ke2);
+}
+
+void launch_elem_galerkin(const DevGrid &g, int l, const double *Kchild, double *Kparent) {
+  const Level &L = g.lv[l], &C = g.lv[l - 1];
+  hipLaunchKernelGGL((k_galerkin<1>), dim3(L.elements()), dim3(576), 0, stream(), L.nelx,
+                     L.nely, L.nelz, C.nely, C.nelz, g.ewy, g.ewz, nullptr, nullptr, Kchild,
+                     Kparent);
+}
+
+// 27-point stencil from element matrices, one thread per (node, neighbour).
+// S[(nb*9 + r*3 + c) * nint + n]; symmetric elimination of fixed DOFs.
+// bc_symmetric = 0 keeps the columns of fixed DOFs and only replaces fixed
+// rows by identity rows (the matrix-free convention, stencil_methods.c:515-518).
+__global__ void k_node_stencil(int nelx, int nely, int nelz, int wy, int wz,
+                               const double *__restrict__ Ke,
+                               const uint8_t *__restrict__ fix, double *__restrict__ S,
+                               int bc_symmetric) {
+  const int64_t nx = nelx + 1, ny = nely + 1, nz = nelz + 1, nint = nx * ny * nz;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= 27 * nint) return;
+  const int nb = (int)(tid / nint);
+  const int64_t n = tid % nint;
+  const int j = (int)(n % ny), k = (int)((n / ny) % nz), i = (int)(n / (ny * nz));
+  const int dx = nb / 9 - 1, dz = (nb / 3) % 3 - 1, dy = nb % 3 - 1;
+  const int i2 = i + dx, j2 = j + dy, k2 = k + dz;
+  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const bool nb_in = i2 >= 0 && i2 < nx && j2 >= 0 && j2 < ny && k2 >= 0 && k2 < nz;
+  if (nb_in) {
+    for (int e = 0; e < 8; e++) {  // elements (0-based) containing node n
+      const int ex = (e & 1) ? 0 : -1, ey = (e & 2) ? 0 : -1, ez = (e & 4) ? 0 : -1;
+      const int ie = i + ex, je = j + ey, ke = k + ez;
+      if (ie < 0 || ie >= nelx || je < 0 || je >= nely || ke < 0 || ke >= nelz) continue;
+      const int ox = dx - ex, oy = dy - ey, oz = dz - ez;  // neighbour offset in element
+      if (ox < 0 || ox > 1 || oy < 0 || oy > 1 || oz < 0 || oz > 1) continue;
+      const int a = cn_of(-ex, -ey, -ez), b = cn_of(ox, oy, oz);
+      const double *K = Ke + 576 * (((int64_t)ie * nelz + ke) * nely + je);
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) v[3 * r + c] += K[24 * (3 * a + r) + 3 * b + c];
+    }
+    const uint8_t m1 = fix[((int64_t)(i + 1) * wz + (k + 1)) * wy + (j + 1)];
+    const uint8_t m2 = fix[((int64_t)(i2 + 1) * wz + (k2 + 1)) * wy + (j2 + 1)];
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 3; c++) {
+        const bool fr = (m1 >> r) & 1, fc = bc_symmetric && ((m2 >> c) & 1);
+        if (fr || fc) v[3 * r + c] = (nb == 13 && r == c) ? 1.0 : 0.0;
+      }
+  }
+  for (int q = 0; q < 9; q++) S[(int64_t)(nb * 9 + q) * nint + n] = v[q];
+}
+
+void launch_node_stencil(const DevGrid &g, int l, const double *Ke, double *S,
+                         int bc_symmetric) {
+  const Level &L = g.lv[l];
+  const int64_t nthreads = 27 * L.interior_nodes();
+  const int bs = 256;
+  hipLaunchKernelGGL(k_node_stencil, dim3((nthreads + bs - 1) / bs), dim3(bs), 0, stream(),
+                     L.nelx, L.nely, L.nelz, L.wy, L.wz, Ke, g.fixmask[l], S, bc_symmetric);
+}
+
+// dense n x n matrix (row-major) of the assembled coarsest level
+__global__ void k_stencil_to_dense(int nx, int ny, int nz, const double *__restrict__ S,
+                                   double *__restrict__ A, int64_t n) {
+  const int64_t nint = (int64_t)nx * ny * nz;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= 27 * nint) return;
+  const int nb = (int)(tid / nint);
+  const int64_t p = tid % nint;
+  const int j = (int)(p % ny), k = (int)((p / ny) % nz), i = (int)(p / ((int64_t)ny * nz));
+  const int i2 = i + nb / 9 - 1, k2 = k + (nb / 3) % 3 - 1, j2 = j + nb % 3 - 1;
+  if (i2 < 0 || i2 >= nx || j2 < 0 || j2 >= ny || k2 < 0 || k2 >= nz) return;
+  const int64_t q = ((int64_t)i2 * nz + k2) * ny + j2;
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++)
+      A[(3 * p + r) * n + 3 * q + c] = S[(int64_t)(nb * 9 + 3 * r + c) * nint + p];
+}
+
+void launch_stencil_to_dense(const DevGrid &g, int l, const double *S, double *A, int n) {
+  const Level &L = g.lv[l];
+  launch_fill<double>((int64_t)n * n, A, 0.0);
+  const int64_t nthreads = 27 * L.interior_nodes();
+  const int bs = 256;
+  hipLaunchKernelGGL(k_stencil_to_dense, dim3((nthreads + bs - 1) / bs), dim3(bs), 0,
+                     stream(), L.nx, L.ny, L.nz, S, A, (int64_t)n);
+}
+
+// ---------------------------------------------------------------------------
+// Coarsest solve z = A^{-1} r with the explicit inverse (row-major, n x n):
+// gather the interior DOFs, one wavefront per row, scatter back to the halo
+// layout (the halo of z stays zero).
+template <typename T>
+__global__ void k_gather_interior(int nx, int ny, int nz, int wy, int wz,
+                                  const T *__restrict__ in, T *__restrict__ out) {
+  const int64_t nint = (int64_t)nx * ny * nz;
+  const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= 3 * nint) return;
+  const int64_t p = d / 3;
+  const int j = (int)(p % ny), k = (int)((p / ny) % nz), i = (int)(p / ((int64_t)ny * nz));
+  out[d] = in[3 * (((int64_t)(i + 1) * wz + (k + 1)) * wy + (j + 1)) + d % 3];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemv_scatter(int nx, int ny, int nz, int wy, int wz,
+                                                      const T *__restrict__ A, int n,
+                                                      const T *__restrict__ x,
+                                                      T *__restrict__ out) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  const T *a = A + (int64_t)row * n;
+  double s = 0.0;
+  for (int c = lane; c < n; c += 64) s += (double)a[c] * (double)x[c];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) {
+    const int64_t p = row / 3;
+    const int j = (int)(p % ny), k = (int)((p / ny) % nz), i = (int)(p / ((int64_t)ny * nz));
+    out[3 * (((int64_t)(i + 1) * wz + (k + 1)) * wy + (j + 1)) + row % 3] = (T)s;
+  }
+}
+
+template <typename T>
+void launch_symv_coarse(const DevGrid &g, int l, const T *Ainv, int n, const T *in, T *out,
+                        T *work) {
+  const Level &L = g.lv[l];
+  const int bs = 256;
+  hipLaunchKernelGGL((k_gather_interior<T>), dim3((n + bs - 1) / bs), dim3(bs), 0, stream(),
+                     L.nx, L.ny, L.nz, L.wy, L.wz, in, work);
+  hipLaunchKernelGGL((k_gemv_scatter<T>), dim3((n + 3) / 4), dim3(256), 0, stream(), L.nx,
+                     L.ny, L.nz, L.wy, L.wz, Ainv, n, work, out);
+}
+
+template void launch_symv_coarse<double>(const DevGrid &, int, const double *, int,
+                                         const double *, double *, double *);
+template void launch_symv_coarse<float>(const DevGrid &, int, const float *, int,
+                                        const float *, float *, float *);
+
+}  // namespace t3d

@@ -1,0 +1,268 @@
+// Design-loop kernels: compliance and sensitivity, the density filter and
+// its transpose, volume, change and the optimality-criteria update.
+#include <algorithm>
+#include <cmath>
+
+#include "t3d_internal.hpp"
+
+namespace t3d {
+
+constexpr int kBS = 256;
+constexpr int kPartials = 1024;
+
+__host__ __device__ constexpr int dn_x(int n) { return (n == 1 || n == 2 || n == 5 || n == 6) ? 1 : 0; }
+__host__ __device__ constexpr int dn_y(int n) { return (n == 0 || n == 1 || n == 4 || n == 5) ? 1 : 0; }
+__host__ __device__ constexpr int dn_z(int n) { return n >= 4 ? 1 : 0; }
+
+__device__ __forceinline__ double blk_sum(double v) {
+  __shared__ double sm[kBS / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < kBS / 64; i++) s += sm[i];
+  __syncthreads();
+  return s;
+}
+
+__device__ __forceinline__ double blk_max(double v) {
+  __shared__ double sm[kBS / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < kBS / 64; i++) s = fmax(s, sm[i]);
+  __syncthreads();
+  return s;
+}
+
+__global__ void k_reduce_final(const double *__restrict__ partials, int n, int op,
+                               double *__restrict__ out) {
+  double s = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x)
+    s = op ? fmax(s, partials[i]) : s + partials[i];
+  s = op ? blk_max(s) : blk_sum(s);
+  if (threadIdx.x == 0) *out = s;
+}
+
+// ---- compliance and sensitivity (stencil_methods.c:808-862) ---------------
+// ce = u_e^T KE u_e; c = sum E(x_e) ce; dc_e = -3 (E0-Emin) x_e^2 ce.
+__global__ __launch_bounds__(kBS) void k_sensitivity(
+    int nelx, int nely, int nelz, int wy, int wz, int few, int fez, double E0, double Emin,
+    const double *__restrict__ KE, const double *__restrict__ x,
+    const double *__restrict__ u, double *__restrict__ dc, double *__restrict__ partials) {
+  const int64_t ne = (int64_t)nelx * nely * nelz;
+  double csum = 0.0;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ne;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int j = (int)(t % nely) + 1, k = (int)((t / nely) % nelz) + 1,
+              i = (int)(t / ((int64_t)nely * nelz)) + 1;
+    double ue[24];
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+      const int64_t nd = 3 * (((int64_t)(i + dn_x(n)) * wz + (k + dn_z(n))) * wy + (j + dn_y(n)));
+      ue[3 * n] = u[nd];
+      ue[3 * n + 1] = u[nd + 1];
+      ue[3 * n + 2] = u[nd + 2];
+    }
+    double ce = 0.0;
+#pragma unroll 4
+    for (int r = 0; r < 24; r++) {
+      double tr = 0.0;
+#pragma unroll
+      for (int c = 0; c < 24; c++) tr += KE[24 * r + c] * ue[c];
+      ce += ue[r] * tr;
+    }
+    const int64_t e = ((int64_t)i * fez + k) * few + j;
+    const double xe = x[e];
+    csum += ce * (Emin + xe * xe * xe * (E0 - Emin));
+    dc[e] = ce * (-3.0 * (E0 - Emin) * xe * xe);
+  }
+  csum = blk_sum(csum);
+  if (threadIdx.x == 0) partials[blockIdx.x] = csum;
+}
+
+void launch_sensitivity(const DevGrid &g, const double *x, const double *u, double *dc,
+                        double *partials, double *c) {
+  const Level &L = g.lv[0];
+  hipLaunchKernelGGL(k_sensitivity, dim3(kPartials), dim3(kBS), 0, stream(), L.nelx, L.nely,
+                     L.nelz, L.wy, L.wz, g.ewy, g.ewz, g.E0, g.Emin, g.KE[0], x, u, dc,
+                     partials);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBS), 0, stream(), partials, kPartials, 0,
+                     c);
+}
+
+// ---- density filter (stencil_optimization.c:228-395) ----------------------
+// neighbourhood i2 in [i1-(ceil(r)-1), i1+ceil(r)] clipped to [1, nel],
+// weight max(0, r - dist).
+struct FilterWin {
+  int lo, hi;
+};
+__device__ __forceinline__ FilterWin fwin(int i1, int cr, int nel) {
+  return FilterWin{max(i1 - (cr - 1), 1), min(i1 + cr + 1, nel + 1)};
+}
+
+// MODE 0: out = (H rho)/Hs; MODE 1: out = v / Hs; MODE 2: out = H v
+template <int MODE>
+__global__ __launch_bounds__(kBS) void k_filter(int nelx, int nely, int nelz, int few,
+                                                int fez, double rmin, int cr,
+                                                const double *__restrict__ in,
+                                                double *__restrict__ out) {
+  const int64_t ne = (int64_t)nelx * nely * nelz;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ne) return;
+  const int j1 = (int)(t % nely) + 1, k1 = (int)((t / nely) % nelz) + 1,
+            i1 = (int)(t / ((int64_t)nely * nelz)) + 1;
+  const FilterWin wi = fwin(i1, cr, nelx), wk = fwin(k1, cr, nelz), wj = fwin(j1, cr, nely);
+  double acc = 0.0, ws = 0.0;
+  for (int i2 = wi.lo; i2 < wi.hi; i2++)
+    for (int k2 = wk.lo; k2 < wk.hi; k2++)
+      for (int j2 = wj.lo; j2 < wj.hi; j2++) {
+        const int d2 = (i1 - i2) * (i1 - i2) + (j1 - j2) * (j1 - j2) + (k1 - k2) * (k1 - k2);
+        const double w = fmax(0.0, rmin - sqrt((double)d2));
+        if (MODE != 1) acc += w * in[((int64_t)i2 * fez + k2) * few + j2];
+        ws += w;
+      }
+  const int64_t e1 = ((int64_t)i1 * fez + k1) * few + j1;
+  if (MODE == 0) out[e1] = acc / ws;
+  if (MODE == 1) out[e1] = in[e1] / ws;
+  if (MODE == 2) out[e1] = acc;
+}
+
+void launch_filter(const DevGrid &g, double rmin, const double *rho, double *out) {
+  const Level &L = g.lv[0];
+  const int64_t ne = L.elements();
+  hipLaunchKernelGGL((k_filter<0>), dim3((ne + kBS - 1) / kBS), dim3(kBS), 0, stream(),
+                     L.nelx, L.nely, L.nelz, g.ewy, g.ewz, rmin, (int)std::ceil(rmin), rho,
+                     out);
+}
+
+void launch_filter_grad(const DevGrid &g, double rmin, const double *v, double *tmp,
+                        double *out) {
+  const Level &L = g.lv[0];
+  const int64_t ne = L.elements();
+  launch_fill<double>(g.nelem, tmp, 0.0);
+  hipLaunchKernelGGL((k_filter<1>), dim3((ne + kBS - 1) / kBS), dim3(kBS), 0, stream(),
+                     L.nelx, L.nely, L.nelz, g.ewy, g.ewz, rmin, (int)std::ceil(rmin), v, tmp);
+  hipLaunchKernelGGL((k_filter<2>), dim3((ne + kBS - 1) / kBS), dim3(kBS), 0, stream(),
+                     L.nelx, L.nely, L.nelz, g.ewy, g.ewz, rmin, (int)std::ceil(rmin), tmp,
+                     out);
+}
+
+// ---- volume (gpu_definitions.c:94-110) ------------------------------------
+__global__ __launch_bounds__(kBS) void k_volume(int nelx, int nely, int nelz, int few, int fez,
+                                                const double *__restrict__ x,
+                                                double *__restrict__ partials) {
+  const int64_t ne = (int64_t)nelx * nely * nelz;
+  double s = 0;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ne;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int j = (int)(t % nely) + 1, k = (int)((t / nely) % nelz) + 1,
+              i = (int)(t / ((int64_t)nely * nelz)) + 1;
+    s += x[((int64_t)i * fez + k) * few + j];
+  }
+  s = blk_sum(s);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+void launch_volume(const DevGrid &g, const double *x, double *partials, double *result) {
+  const Level &L = g.lv[0];
+  hipLaunchKernelGGL(k_volume, dim3(kPartials), dim3(kBS), 0, stream(), L.nelx, L.nely,
+                     L.nelz, g.ewy, g.ewz, x, partials);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBS), 0, stream(), partials, kPartials, 0,
+                     result);
+}
+
+// ---- change = max |x - xnew|, then x = xnew (gpu_definitions.c:112-125) ----
+__global__ __launch_bounds__(kBS) void k_change(int64_t n, double *__restrict__ x,
+                                                const double *__restrict__ xnew,
+                                                double *__restrict__ partials) {
+  double m = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    m = fmax(m, fabs(x[i] - xnew[i]));
+    x[i] = xnew[i];
+  }
+  m = blk_max(m);
+  if (threadIdx.x == 0) partials[blockIdx.x] = m;
+}
+
+void launch_change(int64_t n, double *x, const double *xnew, double *partials, double *result) {
+  hipLaunchKernelGGL(k_change, dim3(kPartials), dim3(kBS), 0, stream(), n, x, xnew, partials);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBS), 0, stream(), partials, kPartials, 1,
+                     result);
+}
+
+// ---- optimality criteria (gpu_definitions.c:127-157) ----------------------
+// Bisection on lambda kept on the device: each step evaluates xnew(lmid) and
+// sum dv (xnew - x) over the padded array, then one thread updates l1/l2.
+// state[0] l1, [1] l2, [2] lmid, [3] done flag, [4] g.
+__global__ __launch_bounds__(kBS) void k_oc_step(int64_t n, const double *__restrict__ x,
+                                                 double *__restrict__ xnew,
+                                                 const double *__restrict__ dv,
+                                                 const double *__restrict__ dc,
+                                                 const double *__restrict__ state,
+                                                 double *__restrict__ partials) {
+  if (state[3] != 0.0) return;
+  const double lmid = 0.5 * (state[1] + state[0]);
+  const double move = 0.2;
+  double s = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double xi = x[i];
+    const double cand = xi * sqrt(-dc[i] / (dv[i] * lmid));
+    const double v = fmax(0.0, fmax(xi - move, fmin(1.0, fmin(xi + move, cand))));
+    xnew[i] = v;
+    s += dv[i] * (v - xi);
+  }
+  s = blk_sum(s);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ void k_oc_bisect(const double *__restrict__ partials, int np,
+                            double *__restrict__ state) {
+  if (state[3] != 0.0) return;
+  double s = 0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) s += partials[i];
+  s = blk_sum(s);
+  if (threadIdx.x == 0) {
+    const double l1 = state[0], l2 = state[1];
+    const double lmid = 0.5 * (l2 + l1);
+    const double gt = s + state[4];
+    double n1 = l1, n2 = l2;
+    if (gt > 0) n1 = lmid; else n2 = lmid;
+    state[0] = n1;
+    state[1] = n2;
+    if (!((n2 - n1) / (n1 + n2) > 1e-6)) state[3] = 1.0;
+  }
+}
+
+// Runs the bisection; the loop condition is checked on the host every 16
+// steps (steps after convergence are skipped on the device).  Returns the
+// number of bisection steps or -1.
+int oc_update(const double *x, double *xnew, const double *dv, const double *dc,
+              int64_t nelem, double gval, double *scratch) {
+  double *state = scratch;          // 8 doubles
+  double *partials = scratch + 8;   // kPartials doubles
+  const double init[8] = {0.0, 1e9, 0.0, 0.0, gval, 0, 0, 0};
+  hipMemcpyAsync(state, init, sizeof(init), hipMemcpyHostToDevice, stream());
+  double h[8];
+  for (int it = 0; it < 4096; it += 16) {
+    for (int q = 0; q < 16; q++) {
+      hipLaunchKernelGGL(k_oc_step, dim3(kPartials), dim3(kBS), 0, stream(), nelem, x, xnew,
+                         dv, dc, state, partials);
+      hipLaunchKernelGGL(k_oc_bisect, dim3(1), dim3(kBS), 0, stream(), partials, kPartials,
+                         state);
+    }
+    hipMemcpyAsync(h, state, sizeof(h), hipMemcpyDeviceToHost, stream());
+    if (!T3D_HIP(hipStreamSynchronize(stream()))) return -1;
+    if (h[3] != 0.0) return it + 16;
+  }
+  return -1;
+}
+
+}  // namespace t3d

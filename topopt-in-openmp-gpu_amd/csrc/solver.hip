@@ -1,0 +1,378 @@
+// MGCG state solve on one GPU: per-solve set-up of every level, the
+// V-cycle preconditioner and the preconditioned CG recurrence
+// (solveStateMG_halo, stencil_solvers.c:129-397).
+//
+// Levels: 0 fine matrix-free, 1 matrix-free with the 8 sub-cell matrices,
+// 2..nl-2 assembled 27-point stencils (symmetric BC elimination), nl-1 the
+// explicit inverse of the FP64 Cholesky-factored coarsest matrix.  Nothing
+// leaves the device inside the CG loop except the residual norm read each
+// iteration for the stopping test.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "solver_state.hpp"
+
+namespace t3d {
+
+template <typename T>
+static T *alloc_t(int64_t n) {
+  return (T *)dalloc(sizeof(T) * (size_t)n);
+}
+
+SolverState *solver_state(const struct SolverDataBuffer *data) {
+  if (!data) {
+    set_error(TOP3D_ERR_ARG, "null SolverDataBuffer");
+    return nullptr;
+  }
+  SolverState *s = (SolverState *)data->bottomSolver.cholmodCommon;
+  if (!s || s->magic != SolverState::kMagic) {
+    set_error(TOP3D_ERR_ARG, "SolverDataBuffer was not allocated by this library");
+    return nullptr;
+  }
+  return s;
+}
+
+// half bandwidth of the coarsest matrix in the x-slowest DOF order
+static int coarse_bandwidth(const Level &L) {
+  return 3 * (L.ny * L.nz + L.ny + 1) + 2;
+}
+
+bool SolverState::allocate(DevGrid *grid, int nlev) {
+  g = grid;
+  nl = nlev;
+  const int64_t n0 = g->lv[0].ndof;
+  r = alloc_t<double>(n0);
+  p = alloc_t<double>(n0);
+  q = alloc_t<double>(n0);
+  z = alloc_t<double>(n0);
+  E64 = alloc_t<double>(g->nelem);
+  E32 = alloc_t<float>(g->nelem);
+  for (int l = 0; l < nl; l++) {
+    const int64_t nd = g->lv[l].ndof;
+    invD64[l] = alloc_t<double>(nd);
+    invD32[l] = alloc_t<float>(nd);
+    for (int b = 0; b < 3; b++) {
+      v64[l][b] = (l == 0 && b == 0) ? nullptr : alloc_t<double>(nd);
+      v32[l][b] = alloc_t<float>(nd);
+    }
+    t64[l] = alloc_t<double>(nd);
+    t32[l] = alloc_t<float>(nd);
+    if (l >= 2) {
+      Ke[l] = alloc_t<double>(576 * g->lv[l].elements());
+      const int64_t sn = 243 * g->lv[l].interior_nodes();
+      S64[l] = alloc_t<double>(sn);
+      S32[l] = alloc_t<float>(sn);
+    }
+  }
+  // level-0 V-cycle vectors in FP64 alias the CG vectors r (input) and z
+  v64[0][kR] = r;
+  hipFree(v64[0][kZ]);
+  v64[0][kZ] = z;
+  const Level &C = g->lv[nl - 1];
+  ncoarse = (int)(3 * C.interior_nodes());
+  bandwidth = coarse_bandwidth(C);
+  const int64_t nn = (int64_t)ncoarse * ncoarse;
+  Ainv64 = alloc_t<double>(nn);
+  Ainv32 = alloc_t<float>(nn);
+  const int nblk = (ncoarse + 63) / 64;
+  dense_work = alloc_t<double>(nn + (int64_t)nblk * 64 * 64 + 8);
+  cwork64 = alloc_t<double>(ncoarse);
+  cwork32 = alloc_t<float>(ncoarse);
+  scal = alloc_t<double>(16);
+  partials = alloc_t<double>(1024 + 16);
+  if (!T3D_HIP(hipHostMalloc((void **)&hscal, sizeof(double) * 16, hipHostMallocDefault)))
+    return false;
+  if (!T3D_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming))) return false;
+  return T3D_HIP(hipStreamSynchronize(stream())) && r && p && q && z && Ainv64 && dense_work;
+}
+
+void SolverState::release() {
+  hipStreamSynchronize(stream());
+  auto f = [](void *ptr) { if (ptr) hipFree(ptr); };
+  for (int l = 0; l < nl; l++) {
+    f(invD64[l]);
+    f(invD32[l]);
+    for (int b = 0; b < 3; b++) {
+      if (!(l == 0 && (b == kR || b == kZ))) f(v64[l][b]);
+      f(v32[l][b]);
+    }
+    f(t64[l]);
+    f(t32[l]);
+    f(Ke[l]);
+    f(S64[l]);
+    f(S32[l]);
+  }
+  f(r); f(p); f(q); f(z); f(E64); f(E32);
+  f(Ainv64); f(Ainv32); f(dense_work); f(cwork64); f(cwork32); f(scal); f(partials);
+  if (hscal) hipHostFree(hscal);
+  if (ev) hipEventDestroy(ev);
+  magic = 0;
+}
+
+// ---------------------------------------------------------------------------
+// per-solve set-up (stencil_solvers.c:283-302)
+bool SolverState::setup(const double *x) {
+  launch_young<double>(*g, x, E64);
+  if (mixed) launch_convert<float, double>(g->nelem, E64, E32);
+  launch_invdiag_mf<double>(*g, 0, E64, invD64[0]);
+  launch_invdiag_mf<double>(*g, 1, E64, invD64[1]);
+  for (int l = 2; l < nl; l++) {
+    if (l == 2)
+      launch_elem_level2(*g, E64, Ke[2]);
+    else
+      launch_elem_galerkin(*g, l, Ke[l - 1], Ke[l]);
+    launch_node_stencil(*g, l, Ke[l], S64[l], 1);
+    launch_invdiag_stencil<double>(*g, l, S64[l], invD64[l]);
+  }
+  launch_stencil_to_dense(*g, nl - 1, S64[nl - 1], Ainv64, ncoarse);
+  const int rc = dense_cholesky_inverse_banded(Ainv64, ncoarse, bandwidth, dense_work);
+  if (rc != TOP3D_OK) return false;
+  if (mixed) {
+    for (int l = 0; l < nl; l++) launch_convert<float, double>(g->lv[l].ndof, invD64[l], invD32[l]);
+    for (int l = 2; l < nl - 1; l++)
+      launch_convert<float, double>(243 * g->lv[l].interior_nodes(), S64[l], S32[l]);
+    launch_convert<float, double>((int64_t)ncoarse * ncoarse, Ainv64, Ainv32);
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// level operator dispatch
+template <typename T>
+void SolverState::op(int l, int mode, const T *in, T *out, const T *b) {
+  const T omega = (T)kOmega;
+  const T *invD = invd<T>(l);
+  if (l == 0)
+    launch_fine<T>(*g, mode, young_t<T>(), in, out, b, invD, omega, bcsym);
+  else if (l == 1)
+    launch_level1<T>(*g, mode, young_t<T>(), in, out, b, invD, omega, bcsym);
+  else
+    launch_stencil<T>(*g, l, mode, stencil_t<T>(l), in, out, b, invD, omega);
+}
+
+// nswp damped-Jacobi sweeps on level l starting from z = 0 (first sweep
+// without an operator application); ping-pong between z and t.  Returns the
+// buffer holding the result.
+template <typename T>
+T *SolverState::smooth_from_zero(int l, const T *b, T *zb, T *tb) {
+  T *cur = (nswp % 2 == 1) ? zb : tb;
+  T *oth = (cur == zb) ? tb : zb;
+  launch_jacobi_zero<T, T>(*g, l, b, invd<T>(l), (T)kOmega, cur);
+  for (int s = 1; s < nswp; s++) {
+    op<T>(l, 1, cur, oth, b);
+    std::swap(cur, oth);
+  }
+  return cur;  // == zb
+}
+
+template <typename T>
+T *SolverState::smooth(int l, const T *b, T *cur, T *oth) {
+  for (int s = 0; s < nswp; s++) {
+    op<T>(l, 1, cur, oth, b);
+    std::swap(cur, oth);
+  }
+  return cur;
+}
+
+// V-cycle at level l: reads vec<T>(l, kR), leaves the result in vec<T>(l, kZ)
+// (stencil_solvers.c:129-252).
+template <typename T>
+void SolverState::vcycle(int l) {
+  T *r_ = vec<T>(l, kR), *z_ = vec<T>(l, kZ), *d_ = vec<T>(l, kD), *t_ = tmp<T>(l);
+  T *zc = smooth_from_zero<T>(l, r_, z_, t_);
+  T *other = (zc == z_) ? t_ : z_;
+  op<T>(l, 2, zc, d_, r_);  // d = r - K z
+  launch_restrict<T>(*g, l, d_, vec<T>(l + 1, kR));
+  if (nl == l + 2) {
+    launch_symv_coarse<T>(*g, l + 1, ainv<T>(), ncoarse, vec<T>(l + 1, kR),
+                          vec<T>(l + 1, kZ), cwork<T>());
+  } else {
+    vcycle<T>(l + 1);
+  }
+  launch_prolong_add<T>(*g, l, vec<T>(l + 1, kZ), zc);  // z += P zc
+  T *res = smooth<T>(l, r_, zc, other);
+  if (res != z_)
+    hipMemcpyAsync(z_, res, sizeof(T) * g->lv[l].ndof, hipMemcpyDeviceToDevice, stream());
+}
+
+void SolverState::precondition() {
+  if (!mixed) {
+    vcycle<double>(0);  // r -> z (aliases)
+    return;
+  }
+  const int64_t n0 = g->lv[0].ndof;
+  launch_convert<float, double>(n0, r, v32[0][kR]);
+  vcycle<float>(0);
+  launch_convert<double, float>(n0, v32[0][kZ], z);
+}
+
+// ---------------------------------------------------------------------------
+// PCG (stencil_solvers.c:259-397) with scalars on the device:
+// scal[0] rho, [1] rho_old, [2] p.q, [3] r.r, [4] b.b
+int SolverState::solve(const double *x, const double *b, double *u, double tol, int *final_iter,
+                       float *final_res) {
+  using clk = std::chrono::steady_clock;
+  const auto t0 = clk::now();
+  stats = Top3dSolveStats{};
+  const int64_t n0 = g->lv[0].ndof;
+  hipMemcpyAsync(z, u, sizeof(double) * n0, hipMemcpyDeviceToDevice, stream());
+  if (!setup(x)) {
+    stats.status = top3d_last_error();
+    return stats.status;
+  }
+  launch_fine<double>(*g, 2, E64, z, r, b, nullptr, 0.0, bcsym);  // r = b - K z
+  launch_dot<double>(b, b, n0, partials, scal + 4);
+  hipMemcpyAsync(hscal + 4, scal + 4, sizeof(double), hipMemcpyDeviceToHost, stream());
+  if (!T3D_HIP(hipStreamSynchronize(stream()))) return TOP3D_ERR_HIP;
+  const auto t1 = clk::now();
+  stats.setup_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  const double bnorm = std::sqrt(hscal[4]);
+  int status = TOP3D_OK;
+  int iter = 0;
+  double relres = 0.0;
+  bool pending = false;  // a V-cycle for iteration `iter` is already queued
+  for (iter = 0; iter < max_iter; iter++) {
+    if (!pending) precondition();
+    pending = false;
+    launch_dot<double>(r, z, n0, partials, scal + 0);
+    launch_pupdate(n0, scal, p, z, iter == 0);
+    launch_fine<double>(*g, 0, E64, p, q, nullptr, nullptr, 0.0, bcsym);
+    launch_dot<double>(p, q, n0, partials, scal + 2);
+    launch_axpy_cg(n0, scal, u, r, p, q);
+    launch_dot<double>(r, r, n0, partials, scal + 3);
+    hipMemcpyAsync(hscal + 3, scal + 3, sizeof(double), hipMemcpyDeviceToHost, stream());
+    hipEventRecord(ev, stream());
+    // queue the next V-cycle while the residual norm travels to the host; if
+    // this iteration converged its result is simply not used
+    if (iter + 1 < max_iter) {
+      precondition();
+      pending = true;
+    }
+    if (!T3D_HIP(hipEventSynchronize(ev))) {
+      status = TOP3D_ERR_HIP;
+      break;
+    }
+    relres = std::sqrt(hscal[3]) / bnorm;
+    if (final_iter) *final_iter = iter;
+    if (final_res) *final_res = (float)relres;
+    if (!std::isfinite(relres)) {
+      status = TOP3D_ERR_NONFINITE;
+      set_error(status, "CG: non-finite residual at iteration %d", iter);
+      break;
+    }
+    if (relres < tol) break;
+  }
+  if (!T3D_HIP(hipStreamSynchronize(stream()))) status = TOP3D_ERR_HIP;
+  if (status == TOP3D_OK && iter >= max_iter) {
+    status = TOP3D_ERR_NO_CONVERGE;
+    set_error(status, "CG: relres %.3e above tol after %d iterations", relres, max_iter);
+    iter = max_iter - 1;
+  }
+  stats.iterations = iter + 1;
+  stats.relres = relres;
+  stats.status = status;
+  stats.solve_ms = std::chrono::duration<double, std::milli>(clk::now() - t1).count();
+  return status;
+}
+
+template void SolverState::vcycle<double>(int);
+template void SolverState::vcycle<float>(int);
+template void SolverState::op<double>(int, int, const double *, double *, const double *);
+template void SolverState::op<float>(int, int, const float *, float *, const float *);
+
+}  // namespace t3d
+
+using namespace t3d;
+
+extern "C" {
+
+// stencil_solvers.c:399-441
+void allocateSolverData(const struct gridContext gc, const int nl,
+                        struct SolverDataBuffer *data) {
+  std::memset(data, 0, sizeof(*data));
+  DevGrid *g = dev_grid(&gc);
+  if (!g) return;
+  if (nl != g->nl) {
+    set_error(TOP3D_ERR_ARG, "allocateSolverData: nl=%d but the grid has %d levels", nl, g->nl);
+    return;
+  }
+  SolverState *s = new SolverState();
+  if (!s->allocate(g, nl)) {
+    s->release();
+    delete s;
+    return;
+  }
+  data->r = s->r;
+  data->p = s->p;
+  data->q = s->q;
+  data->z = s->z;
+  data->invD = (MTYPE **)std::calloc(nl, sizeof(MTYPE *));
+  data->dmg = (CTYPE **)std::calloc(nl, sizeof(CTYPE *));
+  data->rmg = (CTYPE **)std::calloc(nl, sizeof(CTYPE *));
+  data->zmg = (CTYPE **)std::calloc(nl, sizeof(CTYPE *));
+  for (int l = 0; l < nl; l++) {
+    data->invD[l] = s->invD64[l];
+    data->dmg[l] = s->v64[l][SolverState::kD];
+    data->rmg[l] = s->v64[l][SolverState::kR];
+    data->zmg[l] = s->v64[l][SolverState::kZ];
+  }
+  data->coarseMatrices = (struct CSRMatrix *)std::calloc(nl, sizeof(struct CSRMatrix));
+  data->bottomSolver.cholmodCommon = s;
+}
+
+void freeSolverData(struct SolverDataBuffer *data, const int nl) {
+  SolverState *s = solver_state(data);
+  if (!s) return;
+  s->release();
+  delete s;
+  std::free(data->invD);
+  std::free(data->dmg);
+  std::free(data->rmg);
+  std::free(data->zmg);
+  std::free(data->coarseMatrices);
+  std::memset(data, 0, sizeof(*data));
+  (void)nl;
+}
+
+void top3d_set_precision(struct SolverDataBuffer *data, int mode) {
+  SolverState *s = solver_state(data);
+  if (s) s->mixed = (mode == TOP3D_MIXED);
+}
+void top3d_set_bc_mode(struct SolverDataBuffer *data, int mode) {
+  SolverState *s = solver_state(data);
+  if (s) s->bcsym = (mode == TOP3D_BC_SYMMETRIC);
+}
+void top3d_set_max_iter(struct SolverDataBuffer *data, int max_iter) {
+  SolverState *s = solver_state(data);
+  if (s) s->max_iter = max_iter > 0 ? max_iter : 1000;
+}
+void top3d_solver_stats(const struct SolverDataBuffer *data, struct Top3dSolveStats *out) {
+  SolverState *s = solver_state(data);
+  if (s && out) *out = s->stats;
+}
+
+// stencil_solvers.c:259-397
+void solveStateMG_halo(const struct gpuGrid *gpu_gc, DTYPE *x, const int nswp, const int nl,
+                       const CTYPE tol, struct SolverDataBuffer *data, int *finalIter,
+                       float *finalRes, CTYPE *b, STYPE *u) {
+  if (!gpu_gc || gpu_gc->num_targets != 1 || !gpu_gc->target) {
+    set_error(TOP3D_ERR_ARG, "solveStateMG_halo: one GPU target expected");
+    return;
+  }
+  SolverState *s = solver_state(data);
+  if (!s) return;
+  if (nl != s->nl || nswp < 1) {
+    set_error(TOP3D_ERR_ARG, "solveStateMG_halo: nl=%d nswp=%d (solver has %d levels)", nl,
+              nswp, s->nl);
+    return;
+  }
+  const double *dx = (const double *)dptr(x, "solveStateMG_halo x");
+  const double *db = (const double *)dptr(b, "solveStateMG_halo b");
+  double *du = (double *)dptr(u, "solveStateMG_halo u");
+  if (!dx || !db || !du) return;
+  s->nswp = nswp;
+  s->solve(dx, db, du, tol, finalIter, finalRes);
+}
+
+}  // extern "C"

@@ -1,0 +1,85 @@
+// Device-side state of one SolverDataBuffer (kept in its bottomSolver slot).
+#pragma once
+
+#include "t3d_internal.hpp"
+
+namespace t3d {
+
+int dense_cholesky_inverse_banded(double *A, int n, int bw, double *work);
+
+struct SolverState {
+  static constexpr uint32_t kMagic = 0x74336453u;
+  enum { kR = 0, kZ = 1, kD = 2 };
+  uint32_t magic = kMagic;
+  DevGrid *g = nullptr;
+  int nl = 0;
+  int nswp = 4;
+  int mixed = 0;
+  int bcsym = 0;
+  int max_iter = 1000;
+  // outer CG vectors (FP64, level-0 halo layout)
+  double *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr;
+  // Young's moduli of the fine elements
+  double *E64 = nullptr;
+  float *E32 = nullptr;
+  // per level: V-cycle vectors r, z, d (FP64 and FP32), ping-pong buffer
+  double *v64[kMaxLevels][3] = {};
+  float *v32[kMaxLevels][3] = {};
+  double *t64[kMaxLevels] = {};
+  float *t32[kMaxLevels] = {};
+  double *invD64[kMaxLevels] = {};
+  float *invD32[kMaxLevels] = {};
+  // assembled levels: element matrices and 27-point stencils
+  double *Ke[kMaxLevels] = {};
+  double *S64[kMaxLevels] = {};
+  float *S32[kMaxLevels] = {};
+  // coarsest level: explicit inverse
+  int ncoarse = 0, bandwidth = 0;
+  double *Ainv64 = nullptr;
+  float *Ainv32 = nullptr;
+  double *dense_work = nullptr;
+  double *cwork64 = nullptr;
+  float *cwork32 = nullptr;
+  // device scalars, reduction partials, pinned host copy
+  double *scal = nullptr, *partials = nullptr, *hscal = nullptr;
+  hipEvent_t ev = nullptr;
+  Top3dSolveStats stats{};
+
+  bool allocate(DevGrid *grid, int nlev);
+  void release();
+  bool setup(const double *x);
+  void precondition();
+  int solve(const double *x, const double *b, double *u, double tol, int *final_iter,
+            float *final_res);
+
+  template <typename T> void op(int l, int mode, const T *in, T *out, const T *b);
+  template <typename T> void vcycle(int l);
+  template <typename T> T *smooth_from_zero(int l, const T *b, T *zb, T *tb);
+  template <typename T> T *smooth(int l, const T *b, T *cur, T *oth);
+
+  template <typename T> T *vec(int l, int which) {
+    if constexpr (sizeof(T) == 8) return (T *)v64[l][which]; else return (T *)v32[l][which];
+  }
+  template <typename T> T *tmp(int l) {
+    if constexpr (sizeof(T) == 8) return (T *)t64[l]; else return (T *)t32[l];
+  }
+  template <typename T> const T *invd(int l) {
+    if constexpr (sizeof(T) == 8) return (const T *)invD64[l]; else return (const T *)invD32[l];
+  }
+  template <typename T> const T *young_t() {
+    if constexpr (sizeof(T) == 8) return (const T *)E64; else return (const T *)E32;
+  }
+  template <typename T> const T *stencil_t(int l) {
+    if constexpr (sizeof(T) == 8) return (const T *)S64[l]; else return (const T *)S32[l];
+  }
+  template <typename T> const T *ainv() {
+    if constexpr (sizeof(T) == 8) return (const T *)Ainv64; else return (const T *)Ainv32;
+  }
+  template <typename T> T *cwork() {
+    if constexpr (sizeof(T) == 8) return (T *)cwork64; else return (T *)cwork32;
+  }
+};
+
+SolverState *solver_state(const struct SolverDataBuffer *data);
+
+}  // namespace t3d

@@ -1,0 +1,169 @@
+// Internal declarations of the gfx950 MGCG library: level geometry, the
+// device-side grid state, the solver state and the kernel launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/top3d_hip.h"
+
+namespace t3d {
+
+constexpr int kMaxLevels = 8;
+constexpr double kOmega = 0.6;  // stencil_solvers.c:332
+
+// ---------------------------------------------------------------------------
+// error state
+void set_error(int code, const char *fmt, ...);
+bool hip_ok(hipError_t e, const char *what);
+#define T3D_HIP(call) ::t3d::hip_ok((call), #call)
+
+hipStream_t stream();
+
+// ---------------------------------------------------------------------------
+// Geometry of one multigrid level in the halo layout: node (i,j,k) with
+// i in [0,wx), j in [0,wy), k in [0,wz) at i*wy*wz + k*wy + j; real nodes are
+// 1..nel+1 per direction, DOF = 3*node + comp.
+struct Level {
+  int l = 0, ncell = 1;
+  int nelx = 0, nely = 0, nelz = 0;
+  int nx = 0, ny = 0, nz = 0;
+  int wx = 0, wy = 0, wz = 0;
+  int64_t ndof = 0;
+  int64_t nodes() const { return (int64_t)wx * wy * wz; }
+  int64_t interior_nodes() const { return (int64_t)nx * ny * nz; }
+  int64_t elements() const { return (int64_t)nelx * nely * nelz; }
+};
+
+// Kernel argument bundle for a level plus the fine element layout.
+struct LevelArgs {
+  int nelx, nely, nelz;  // level elements
+  int wy, wz;            // level node strides
+  int ncell;             // fine elements per level element per direction
+  int few, fez;          // fine element strides: e = i*few*fez + k*few + j
+};
+
+// Device-side state of a gridContext (keyed by gc->precomputedKE).
+struct DevGrid {
+  int nl = 0;
+  int nelx = 0, nely = 0, nelz = 0;
+  Level lv[kMaxLevels];
+  int ewy = 0, ewz = 0;   // fine element strides (wrapy-1, wrapz-1)
+  int64_t nelem = 0;      // padded fine element count (wrapx-1)(wrapy-1)(wrapz-1)
+  double E0 = 1.0, Emin = 1e-6;
+  uint8_t *fixmask[kMaxLevels] = {};   // per halo node: bit c set = DOF fixed
+  int64_t nfixed[kMaxLevels] = {};
+  int64_t *fixed[kMaxLevels] = {};     // fixed DOF indices (device)
+  double *KE[2] = {};                  // device KE_0 (576), KE_1 (8*576)
+  float *KEf[2] = {};
+  void *scratch[8] = {};               // lazily allocated work arrays of the API
+  size_t scratch_bytes[8] = {};
+  LevelArgs args(int l) const;
+  void *work(int slot, size_t bytes);  // scratch[slot], grown to >= bytes
+};
+
+DevGrid *dev_grid(const struct gridContext *gc);  // nullptr + error if absent
+DevGrid *dev_grid_node(const struct gpuNode *node);
+void release_dev_grid(const struct gridContext *gc);
+void compute_level(const struct gridContext *gc, int l, Level *out);
+
+// present table
+void *dptr(const void *p, const char *what);  // mapped host or device pointer
+bool is_device_ptr(const void *p);
+
+// device allocation helpers (zero-filled)
+void *dalloc(size_t bytes);
+void dfree(void *p);
+
+// ---------------------------------------------------------------------------
+// Kernel launchers.  T is double or float; all run on stream().
+template <typename T>
+void launch_young(const DevGrid &g, const double *x, T *E);
+template <typename T>
+void launch_young_float_input(const DevGrid &g, const float *x, T *E);
+
+// fine level: mode 0 out = K in; 1 Jacobi sweep out = in + w invD (b - K in)
+// (interior nodes; other nodes copied from in); 2 residual out = b - K in
+template <typename T>
+void launch_fine(const DevGrid &g, int mode, const T *E, const T *in, T *out,
+                 const T *b, const T *invD, T omega, int bcsym);
+// level-1 matrix-free operator with fine moduli, same modes
+template <typename T>
+void launch_level1(const DevGrid &g, int mode, const T *E, const T *in, T *out,
+                   const T *b, const T *invD, T omega, int bcsym);
+// assembled 27-point node stencil (levels >= 2), same modes
+template <typename T>
+void launch_stencil(const DevGrid &g, int l, int mode, const T *S, const T *in,
+                    T *out, const T *b, const T *invD, T omega);
+// first pre-smoothing sweep from z = 0:  out = w invD b on interior, else 0
+template <typename TB, typename T>
+void launch_jacobi_zero(const DevGrid &g, int l, const TB *b, const T *invD,
+                        T omega, T *out);
+template <typename T>
+void launch_restrict(const DevGrid &g, int l, const T *fine, T *coarse);
+template <typename T>
+void launch_prolong_add(const DevGrid &g, int l, const T *coarse, T *fine);
+template <typename T>
+void launch_prolong(const DevGrid &g, int l, const T *coarse, T *fine);
+// inverse diagonal of level l (0, 1: from moduli; >= 2: from a stencil)
+template <typename T>
+void launch_invdiag_mf(const DevGrid &g, int l, const double *E, T *invD);
+template <typename T>
+void launch_invdiag_stencil(const DevGrid &g, int l, const double *S, T *invD);
+
+// coarse element matrices and stencils (FP64)
+void launch_elem_level2(const DevGrid &g, const double *E, double *Ke2);
+void launch_elem_galerkin(const DevGrid &g, int l, const double *Kchild,
+                          double *Kparent);
+void launch_node_stencil(const DevGrid &g, int l, const double *Ke, double *S,
+                         int bc_symmetric);
+void launch_stencil_to_dense(const DevGrid &g, int l, const double *S,
+                             double *A, int n);
+
+// dense FP64 coarse solver
+int dense_cholesky_inverse(double *A, int n, double *work);  // A <- A^{-1}
+template <typename T>
+void launch_symv_coarse(const DevGrid &g, int l, const T *Ainv, int n,
+                        const T *in, T *out, T *work);
+
+// BLAS-1 (FP64 accumulation in the reductions)
+template <typename T>
+void launch_dot(const T *a, const T *b, int64_t n, double *partials,
+                double *result);
+void launch_axpy_cg(int64_t n, double *scal, double *u, double *r, const double *p,
+                    const double *q);
+template <typename TO, typename TI>
+void launch_convert(int64_t n, const TI *in, TO *out);
+template <typename T>
+void launch_fill(int64_t n, T *out, T v);
+template <typename T>
+void launch_sub(int64_t n, const T *a, const T *b, T *out);  // out = a - b
+template <typename T>
+void launch_add(int64_t n, const T *a, T *out);               // out += a
+void launch_pupdate(int64_t n, const double *scal, double *p, const double *z,
+                    int first);
+void launch_copy_fixed(const DevGrid &g, int l, const double *in, double *out);
+
+// design-loop kernels
+void launch_sensitivity(const DevGrid &g, const double *x, const double *u,
+                        double *dc, double *partials, double *c);
+void launch_filter(const DevGrid &g, double rmin, const double *rho,
+                   double *out);
+void launch_filter_grad(const DevGrid &g, double rmin, const double *v,
+                        double *tmp, double *out);
+void launch_volume(const DevGrid &g, const double *x, double *partials,
+                   double *result);
+void launch_change(int64_t n, double *x, const double *xnew, double *partials,
+                   double *result);
+int oc_update(const double *x, double *xnew, const double *dv,
+              const double *dc, int64_t nelem, double gval, double *scratch);
+
+// ---------------------------------------------------------------------------
+// Host-side KE generation (same quadrature results as local_matrix.c).
+void element_matrices(double nu, int l, double *KE);
+
+}  // namespace t3d
