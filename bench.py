@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""MGCG state-solve benchmark on one MI355X.
+
+Metric (the reference's headline path): seconds of MGCG state solve per
+SIMP design iteration on the 256x128x128-element cantilever (coarse 16x8x8,
+5 multigrid levels, rtol 1e-5, V(4,4) damped Jacobi, omega 0.6), mixed
+precision (FP32 V-cycle, FP64 outer CG and coarse Cholesky), next to the
+all-FP64 form on the same run; plus the achieved HBM bandwidth of the
+dominant stencil kernel.
+
+A step is one design iteration of top3dmgcg (stencil_optimization.c:150-189)
+driven through the C ABI: solve, volume, sensitivity, filter gradient, OC
+update, change, filter.  W untimed warm-up steps, then exactly K timed steps
+between device synchronisations.  Data are the synthetic cantilever (load and
+supports of the reference), uniform initial density 0.2.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "topopt-in-openmp-gpu_amd"))
+
+HBM_PEAK_GBS = 8000.0
+SWEEPS_PER_VCYCLE = 7  # 4 pre (the first from zero needs no operator) + 4 post - 1
+
+
+def run_form(top3d_amd, torch, nelx, nely, nelz, nl, precision, warmup, steps):
+    g = top3d_amd.Grid(nelx, nely, nelz, nl, precision=precision)
+    st = top3d_amd.DesignState(g, volfrac=0.2, rmin=1.5)
+    recs = []
+
+    def step():
+        t0 = time.perf_counter()
+        rec = top3d_amd.design_step(st, cgtol=1e-5)
+        rec["solve_s"] = g.stats()["solve_ms"] / 1e3 + g.stats()["setup_ms"] / 1e3
+        rec["step_s"] = time.perf_counter() - t0
+        return rec
+
+    for _ in range(warmup):
+        r = step()
+        print(f"[bench] {precision} warm-up step: solve {r['solve_s']:.4f}s cg {r['cgiter'] + 1}",
+              file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        recs.append(step())
+        print(f"[bench] {precision} step {len(recs)}: solve {recs[-1]['solve_s']:.4f}s "
+              f"cg {recs[-1]['cgiter'] + 1} c {recs[-1]['c']:.6e}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return g, st, recs, wall
+
+
+def cpu_baseline(nelx, nely, nelz, nl, gpu_iters):
+    """FP64 oracle on the host: per-solve set-up plus one CG iteration of the
+    first design iteration, extrapolated to the GPU's CG iteration count."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    g = pyoracle.Grid(nelx, nely, nelz, nl)
+    F = g.load()
+    _, wx, wy, wz, ndof = g.padding(0)
+    x = np.zeros(g.nelem())
+    i, k, j = np.meshgrid(np.arange(1, nelx + 1), np.arange(1, nelz + 1),
+                          np.arange(1, nely + 1), indexing="ij")
+    x[(i * (wy - 1) * (wz - 1) + k * (wy - 1) + j).ravel()] = float(np.float32(0.2))
+    U = np.zeros(ndof)
+    g.solve(x, F, U, maxiter=1)
+    setup_s, loop_s = g.solve_times()
+    g.close()
+    value = setup_s + loop_s * gpu_iters
+    return {"value": value, "unit": "s/design-iter", "cores": threads, "kind": "port",
+            "sample": (f"FP64 oracle (oracle/top3d_oracle.c, OpenMP {threads} threads) on the same "
+                       f"{nelx}x{nely}x{nelz} nl={nl} cantilever from uniform density: per-solve "
+                       f"set-up {setup_s:.2f}s + 1 CG iteration {loop_s:.2f}s measured, "
+                       f"extrapolated to {gpu_iters:.1f} CG iterations per solve (the GPU FP64 "
+                       f"run's mean)")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--coarse", default="16,8,8")
+    ap.add_argument("--levels", type=int, default=5)
+    ap.add_argument("--compare-fp64", type=int, default=1)
+    ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--kernel-reps", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU; --gpus N > 1 is not supported "
+              "(the x-slab multi-GPU decomposition is design-only)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import top3d_amd
+
+    cx, cy, cz = (int(v) for v in args.coarse.split(","))
+    f = 1 << (args.levels - 1)
+    nelx, nely, nelz, nl = cx * f, cy * f, cz * f, args.levels
+    dofs = 3 * (nelx + 1) * (nely + 1) * (nelz + 1)
+
+    g, st, recs, wall = run_form(top3d_amd, torch, nelx, nely, nelz, nl, "mixed",
+                                 args.warmup, args.steps)
+    solve_s = sum(r["solve_s"] for r in recs) / len(recs)
+    cg_mixed = [r["cgiter"] + 1 for r in recs]
+
+    # kernel timings on the mixed run's buffers (HIP events, library stream)
+    kernels = {}
+    for name in ("fine_sweep_f32", "level1_sweep_f32", "fine_apply_f64", "fine_sweep_f64"):
+        ms, by = g.time_kernel(name, args.kernel_reps)
+        kernels[name] = {"avg_ms": ms, "bytes": by, "GB_s": by / (ms * 1e-3) / 1e9,
+                         "hbm_frac": by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    # dominant kernel of the mixed solve: the larger per-CG-iteration time
+    dom = max(("fine_sweep_f32", "level1_sweep_f32"),
+              key=lambda n: kernels[n]["avg_ms"] * SWEEPS_PER_VCYCLE)
+    g.close()
+
+    fp64 = None
+    if args.compare_fp64:
+        g64, _, recs64, _ = run_form(top3d_amd, torch, nelx, nely, nelz, nl, "fp64",
+                                     args.warmup, args.steps)
+        s64 = sum(r["solve_s"] for r in recs64) / len(recs64)
+        cdiff = max(abs(a["c"] - b["c"]) / abs(b["c"]) for a, b in zip(recs, recs64))
+        fp64 = {"solve_s_per_design_iter": s64,
+                "cg_iters": [r["cgiter"] + 1 for r in recs64],
+                "compliance": [r["c"] for r in recs64],
+                "mixed_vs_fp64_max_rel_compliance_diff": cdiff,
+                "speedup_mixed_over_fp64": s64 / solve_s}
+        g64.close()
+
+    cpu = None
+    if args.cpu_baseline:
+        iters = (sum(fp64["cg_iters"]) / len(fp64["cg_iters"]) if fp64
+                 else sum(cg_mixed) / len(cg_mixed))
+        print("[bench] CPU baseline (oracle) ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(nelx, nely, nelz, nl, iters)
+
+    k = kernels[dom]
+    out = {
+        "metric": "MGCG solve s/design-iter at 256x128x128, mixed vs FP64 @rtol 1e-5; matvec GB/s",
+        "value": solve_s,
+        "unit": "s/design-iter",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 V-cycle / f64 CG",
+        "data": "synthetic cantilever (reference load and supports), uniform initial density 0.2",
+        "config": {"workload": f"SIMP design iterations, MGCG state solve, cantilever "
+                               f"{nelx}x{nely}x{nelz} elements, coarse {cx}x{cy}x{cz}, "
+                               f"{nl} levels, V(4,4), rtol 1e-5",
+                   "elements": nelx * nely * nelz, "dofs": dofs, "levels": nl,
+                   "precision": "mixed"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": k["GB_s"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": k["GB_s"] / HBM_PEAK_GBS,
+                     "traffic": None},
+        "cpu_baseline": cpu,
+        "cg_iters": cg_mixed,
+        "compliance": [r["c"] for r in recs],
+        "design_iter_s": wall / args.steps,
+        "dofs_per_s": dofs / solve_s,
+        "kernels": kernels,
+        "fp64": fp64,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

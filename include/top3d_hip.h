@@ -304,6 +304,19 @@ int top3d_compliance_backward(const struct gpuGrid *gpu_gc, double rmin,
                               const double *xPhys_dev, const double *U_dev,
                               double grad_c, double *grad_x_dev);
 
+/* Times `reps` back-to-back launches of one hot kernel on the buffers of the
+ * last solve (HIP events on the library stream).  `which`:
+ *   0 fine damped-Jacobi sweep, FP32 (mixed V-cycle)
+ *   1 fine damped-Jacobi sweep, FP64
+ *   2 fine operator, FP64 (the outer CG matvec q = K p)
+ *   3 level-1 damped-Jacobi sweep, FP32
+ *   4 level-1 damped-Jacobi sweep, FP64
+ * Outputs the mean milliseconds per launch and the algorithmic HBM bytes of
+ * one launch (true nodes and elements only: vectors read/written once,
+ * moduli read once).                                                         */
+int top3d_time_kernel(struct SolverDataBuffer *data, int which, int reps,
+                      double *avg_ms, double *bytes);
+
 /* The coarsest-level dense solver on its own: A (device, n x n row-major,
  * symmetric positive definite, nonzero only within |i-j| <= bw) is replaced
  * by A^{-1} (banded Cholesky on FP64 MFMA, then banded solves).  Returns

@@ -262,6 +262,13 @@ class Grid:
             raise RuntimeError("oracle coarse factorization failed")
         return it.value, res.value
 
+    def solve_times(self):
+        """(set-up seconds, CG-loop seconds) of the last solve."""
+        a, b = c_double(), c_double()
+        self.lib.or_solver_times.argtypes = [c_void_p, c_void_p, c_void_p]
+        self.lib.or_solver_times(self.solver(), ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
     def vcycle(self, x, r):
         """One V-cycle z = M^{-1} r at level 0 (setup must have run)."""
         s = self.solver()

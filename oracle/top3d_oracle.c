@@ -146,6 +146,7 @@ void or_ke(double nu, int l, int npts, double *KE) {
   const double spacing = 2.0 / (double)ncell / (double)npts;
   const double vol = spacing * spacing * spacing;
   memset(KE, 0, sizeof(double) * 576 * ncell * ncell * ncell);
+#pragma omp parallel for collapse(3) schedule(dynamic)
   for (int ii = 0; ii < ncell; ii++)
     for (int kk = 0; kk < ncell; kk++)
       for (int jj = 0; jj < ncell; jj++) {
@@ -370,9 +371,15 @@ void or_matvec_l(const or_grid *g, int l, const or_dtype *x,
   or_ctype *mbuf;
   const or_ctype *inm = masked_input(g, l, in, ndof, &mbuf);
   for (int64_t i = 0; i < ndof; i++) out[i] = 0;
-  for (int i = 1; i < nelxc + 1; i++)
-    for (int k = 1; k < nelzc + 1; k++)
-      for (int j = 1; j < nelyc + 1; j++) {
+  /* 8-colour element loop (stencil_methods.c:457-459): elements of one
+   * colour share no node, so each colour is a parallel loop */
+  for (int bx = 0; bx < 2; bx++)
+    for (int bz = 0; bz < 2; bz++)
+      for (int by = 0; by < 2; by++)
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int i = 1 + bx; i < nelxc + 1; i += 2)
+    for (int k = 1 + bz; k < nelzc + 1; k += 2)
+      for (int j = 1 + by; j < nelyc + 1; j += 2) {
         int64_t edof[8];
         for (int n = 0; n < 8; n++)
           edof[n] = 3 * ((int64_t)(i + lnode[n][0]) * wy * wz +
@@ -410,6 +417,7 @@ void or_prolong(const or_grid *g, int l, const or_ctype *coarse,
   const int ncf = 1 << l;
   const int nxf = g->nelx / ncf + 1, nyf = g->nely / ncf + 1,
             nzf = g->nelz / ncf + 1;
+#pragma omp parallel for collapse(2) schedule(static)
   for (int i = 1; i < nxf + 1; i++)
     for (int k = 1; k < nzf + 1; k++)
       for (int j = 1; j < nyf + 1; j++) {
@@ -441,6 +449,7 @@ void or_restrict(const or_grid *g, int l, const or_ctype *fine,
   const int nxc = g->nelx / ncc + 1, nyc = g->nely / ncc + 1,
             nzc = g->nelz / ncc + 1;
   const double vals[4] = {1.0, 0.5, 0.25, 0.125};
+#pragma omp parallel for collapse(2) schedule(static)
   for (int ic = 1; ic < nxc + 1; ic++)
     for (int kc = 1; kc < nzc + 1; kc++)
       for (int jc = 1; jc < nyc + 1; jc++) {
@@ -477,9 +486,13 @@ void or_invdiag(const or_grid *g, const or_dtype *x, int l, double *diag) {
   const double *KE = g->KE[l];
   const int64_t ewy = fwy - 1, ewz = fwz - 1;
   for (int64_t i = 0; i < ndof; i++) diag[i] = 0.0;
-  for (int i = 1; i < nelxc + 1; i++)
-    for (int k = 1; k < nelzc + 1; k++)
-      for (int j = 1; j < nelyc + 1; j++)
+  for (int bx = 0; bx < 2; bx++)
+    for (int bz = 0; bz < 2; bz++)
+      for (int by = 0; by < 2; by++)
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int i = 1 + bx; i < nelxc + 1; i += 2)
+    for (int k = 1 + bz; k < nelzc + 1; k += 2)
+      for (int j = 1 + by; j < nelyc + 1; j += 2)
         for (int ii = 0; ii < ncell; ii++)
           for (int kk = 0; kk < ncell; kk++)
             for (int jj = 0; jj < ncell; jj++) {
@@ -577,10 +590,15 @@ void or_csr_assemble(const or_grid *g, int l, const or_dtype *x, or_csr *M,
   const int nyc = nelyc + 1, nzc = nelzc + 1;
   const int64_t ewy = fwy - 1, ewz = fwz - 1;
   for (int64_t i = 0; i < M->nnz; i++) M->vals[i] = 0.0;
-  double ke[24][24];
-  for (int i = 0; i < nelxc; i++)
-    for (int k = 0; k < nelzc; k++)
-      for (int j = 0; j < nelyc; j++) {
+  /* 8-colour element loop, stencil_assembly.c:150-155 */
+  for (int bx = 0; bx < 2; bx++)
+    for (int bz = 0; bz < 2; bz++)
+      for (int by = 0; by < 2; by++)
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int i = bx; i < nelxc; i += 2)
+    for (int k = bz; k < nelzc; k += 2)
+      for (int j = by; j < nelyc; j += 2) {
+        double ke[24][24];
         int64_t edof[24];
         for (int n = 0; n < 8; n++)
           for (int c = 0; c < 3; c++)
@@ -630,6 +648,7 @@ void or_csr_apply(const or_grid *g, int l, const or_csr *M,
   const int nxc = g->nelx / ncell + 1, nyc = g->nely / ncell + 1,
             nzc = g->nelz / ncell + 1;
   for (int64_t i = 0; i < ndof; i++) out[i] = 0;
+#pragma omp parallel for collapse(2) schedule(static)
   for (int i = 1; i < nxc + 1; i++)
     for (int k = 1; k < nzc + 1; k++)
       for (int j = 1; j < nyc + 1; j++) {
@@ -712,22 +731,30 @@ static int coarse_factor(or_solver *s, const or_csr *M) {
   free(s->chol);
   s->chol = (double *)calloc((size_t)n * n, sizeof(double));
   double *A = s->chol;
+  int bw = 0;
   for (int r = 0; r < n; r++)
     for (int c = M->rowOffsets[r]; c < M->rowOffsets[r + 1]; c++) {
       const int col = M->colIndex[c];
-      if (col <= r) A[(size_t)r * n + col] = M->vals[c];
+      if (col <= r) {
+        A[(size_t)r * n + col] = M->vals[c];
+        if (r - col > bw) bw = r - col;
+      }
     }
+  s->bw = bw;
+  /* the factor keeps the band of A (no fill outside |i-j| <= bw) */
   for (int j = 0; j < n; j++) {
+    const int k0 = j - bw > 0 ? j - bw : 0;
     double d = A[(size_t)j * n + j];
-    for (int k = 0; k < j; k++) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
+    for (int k = k0; k < j; k++) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
     if (!(d > 0.0)) return -1;
     d = sqrt(d);
     A[(size_t)j * n + j] = d;
-#pragma omp parallel for schedule(static)
-    for (int i = j + 1; i < n; i++) {
+    const int i1 = j + bw < n - 1 ? j + bw : n - 1;
+#pragma omp parallel for schedule(static) if (i1 - j > 64)
+    for (int i = j + 1; i <= i1; i++) {
       double v = A[(size_t)i * n + j];
       const double *ri = A + (size_t)i * n, *rj = A + (size_t)j * n;
-      for (int k = 0; k < j; k++) v -= ri[k] * rj[k];
+      for (int k = i - bw > 0 ? i - bw : 0; k < j; k++) v -= ri[k] * rj[k];
       A[(size_t)i * n + j] = v / d;
     }
   }
@@ -751,14 +778,16 @@ void or_coarse_solve(const or_grid *g, const or_solver *s, int l,
         for (int c = 0; c < 3; c++) y[3 * t + c] = in[3 * h + c];
       }
   const double *L = s->chol;
+  const int bw = s->bw;
   for (int i = 0; i < n; i++) {
     double v = y[i];
-    for (int k = 0; k < i; k++) v -= L[(size_t)i * n + k] * y[k];
+    for (int k = i - bw > 0 ? i - bw : 0; k < i; k++) v -= L[(size_t)i * n + k] * y[k];
     y[i] = v / L[(size_t)i * n + i];
   }
   for (int i = n - 1; i >= 0; i--) {
     double v = y[i];
-    for (int k = i + 1; k < n; k++) v -= L[(size_t)k * n + i] * y[k];
+    const int k1 = i + bw < n - 1 ? i + bw : n - 1;
+    for (int k = i + 1; k <= k1; k++) v -= L[(size_t)k * n + i] * y[k];
     y[i] = v / L[(size_t)i * n + i];
   }
   for (int i = 1; i < nxc + 1; i++)
@@ -847,6 +876,7 @@ void or_vcycle(const or_grid *g, or_solver *s, const or_dtype *x, int l,
   const int nl = g->nl;
   const int64_t nd = level_ndof(g, l), ndn = level_ndof(g, l + 1);
   or_ctype *z = s->zmg[l], *d = s->dmg[l], *r = s->rmg[l];
+#pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < nd; i++) z[i] = 0;
   if (l == 0) {
     or_jacobi0(g, x, nswp, omega, s->invD[0], z, r, d);
@@ -858,6 +888,7 @@ void or_vcycle(const or_grid *g, or_solver *s, const or_dtype *x, int l,
     jacobi_csr(g, l, &s->csr[l], nswp, omega, s->invD[l], z, r, d);
     or_csr_apply(g, l, &s->csr[l], z, d);
   }
+#pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < nd; i++) d[i] = r[i] - d[i];
   or_restrict(g, l, d, s->rmg[l + 1]);
   if (nl == l + 2) {
@@ -867,6 +898,7 @@ void or_vcycle(const or_grid *g, or_solver *s, const or_dtype *x, int l,
     or_vcycle(g, s, x, l + 1, omega, nswp);
   }
   or_prolong(g, l, s->zmg[l + 1], d);
+#pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < nd; i++) z[i] += d[i];
   if (l == 0)
     or_jacobi0(g, x, nswp, omega, s->invD[0], z, r, d);
@@ -876,55 +908,85 @@ void or_vcycle(const or_grid *g, or_solver *s, const or_dtype *x, int l,
     jacobi_csr(g, l, &s->csr[l], nswp, omega, s->invD[l], z, r, d);
 }
 
-/* Reductions, stencil_solvers.h:36-58 (accumulated in CTYPE). */
-static or_ctype norm2(const or_ctype *v, int64_t n) {
-  or_ctype s = 0;
-  for (int64_t i = 0; i < n; i++) s += v[i] * v[i];
-  return (or_ctype)sqrt(s);
-}
+/* Reductions, stencil_solvers.h:36-58 (accumulated in CTYPE).  Chunks of a
+ * fixed size are summed in parallel and the chunk sums in order, so the
+ * result does not depend on the thread count.                               */
+#define OR_CHUNK 8192
 static or_ctype dot(const or_ctype *a, const or_ctype *b, int64_t n) {
+  const int64_t nch = (n + OR_CHUNK - 1) / OR_CHUNK;
+  or_ctype *part = (or_ctype *)malloc(sizeof(or_ctype) * (nch > 0 ? nch : 1));
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < nch; c++) {
+    or_ctype s = 0;
+    const int64_t e = (c + 1) * OR_CHUNK < n ? (c + 1) * OR_CHUNK : n;
+    for (int64_t i = c * OR_CHUNK; i < e; i++) s += a[i] * b[i];
+    part[c] = s;
+  }
   or_ctype s = 0;
-  for (int64_t i = 0; i < n; i++) s += a[i] * b[i];
+  for (int64_t c = 0; c < nch; c++) s += part[c];
+  free(part);
   return s;
 }
+static or_ctype norm2(const or_ctype *v, int64_t n) {
+  return (or_ctype)sqrt(dot(v, v, n));
+}
+
+#include <omp.h>
 
 /* MGCG, solveStateMG_halo, stencil_solvers.c:259-397.  maxIter is 1000 in
- * the reference (:334); a smaller value bounds a CPU timing sample.
+ * the reference (:334); a smaller value bounds a CPU timing sample.  The
+ * set-up and CG-loop wall times are left in s->setup_s and s->loop_s.
  * Returns 0, or -1 when the coarse factorization fails.                    */
 int or_solve(const or_grid *g, or_solver *s, or_dtype *x, int nswp,
              or_ctype tol, int *finalIter, float *finalRes, or_ctype *b,
              or_stype *u, int maxIter) {
+  const double t0 = omp_get_wtime();
   const int64_t ndof = level_ndof(g, 0);
   or_ctype *r = s->r, *p = s->p, *q = s->q, *z = s->z;
+#pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < ndof; i++) z[i] = (or_ctype)u[i];
   if (or_solver_setup(g, s, x)) return -1;
   or_ctype rhoold = 0, dpr, alpha, rho;
   or_matvec0(g, x, z, r);
+#pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < ndof; i++) r[i] = b[i] - r[i];
   const double omega = 0.6;
   const or_ctype bnorm = norm2(b, ndof);
+  const double t1 = omp_get_wtime();
+  s->setup_s = t1 - t0;
   for (int iter = 0; iter < maxIter; iter++) {
     or_vcycle(g, s, x, 0, (or_ctype)omega, nswp);
     rho = dot(r, z, ndof);
     if (iter == 0) {
+#pragma omp parallel for schedule(static)
       for (int64_t i = 0; i < ndof; i++) p[i] = z[i];
     } else {
       const or_ctype beta = rho / rhoold;
+#pragma omp parallel for schedule(static)
       for (int64_t i = 0; i < ndof; i++) p[i] = beta * p[i] + z[i];
     }
     or_matvec0(g, x, p, q);
     dpr = dot(p, q, ndof);
     alpha = rho / dpr;
     rhoold = rho;
-    for (int64_t i = 0; i < ndof; i++) u[i] += (or_stype)(alpha * p[i]);
-    for (int64_t i = 0; i < ndof; i++) r[i] -= alpha * q[i];
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < ndof; i++) {
+      u[i] += (or_stype)(alpha * p[i]);
+      r[i] -= alpha * q[i];
+    }
     const or_ctype rnorm = norm2(r, ndof);
     const or_ctype relres = rnorm / bnorm;
     *finalIter = iter;
     *finalRes = (float)relres;
     if (relres < tol) break;
   }
+  s->loop_s = omp_get_wtime() - t1;
   return 0;
+}
+
+void or_solver_times(const or_solver *s, double *setup_s, double *loop_s) {
+  *setup_s = s->setup_s;
+  *loop_s = s->loop_s;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -938,33 +1000,46 @@ void or_sensitivity(const or_grid *g, const or_dtype *x, const or_stype *u,
   const int64_t ewy = wy - 1, ewz = wz - 1;
   const double *KE = g->KE[0];
   const double E_diff = g->E0 - g->Emin, Emin = g->Emin;
-  for (int i = 1; i < g->nelx + 1; i++)
-    for (int k = 1; k < g->nelz + 1; k++)
-      for (int j = 1; j < g->nely + 1; j++) dcdx[i * ewy * ewz + k * ewy + j] = 0;
+  const int64_t nelx = g->nelx, nely = g->nely, nelz = g->nelz;
+  const int64_t ne = nelx * nely * nelz;
+  const int64_t nch = (ne + OR_CHUNK - 1) / OR_CHUNK;
+  or_dtype *part = (or_dtype *)malloc(sizeof(or_dtype) * (nch > 0 ? nch : 1));
+#pragma omp parallel for schedule(static)
+  for (int64_t t = 0; t < ne; t++) {
+    const int64_t i = t / (nely * nelz) + 1, k = (t / nely) % nelz + 1, j = t % nely + 1;
+    dcdx[i * ewy * ewz + k * ewy + j] = 0;
+  }
   or_dtype cc = 0;
-  for (int ii = 0; ii < 8; ii++)
-    for (int i = 1; i < g->nelx + 1; i++)
-      for (int k = 1; k < g->nelz + 1; k++)
-        for (int j = 1; j < g->nely + 1; j++) {
-          int64_t edof[8];
-          for (int n = 0; n < 8; n++)
-            edof[n] = 3 * ((int64_t)(i + lnode[n][0]) * wy * wz +
-                           (k + lnode[n][2]) * wy + (j + lnode[n][1]));
-          const int64_t e = i * ewy * ewz + k * ewy + j;
-          or_stype cl = 0.0;
-          for (int a = 0; a < 3; a++) {
-            double t = 0.0;
-            for (int n = 0; n < 8; n++) {
-              t += KE[(ii * 3 + a) * 24 + 3 * n] * u[edof[n]];
-              t += KE[(ii * 3 + a) * 24 + 3 * n + 1] * u[edof[n] + 1];
-              t += KE[(ii * 3 + a) * 24 + 3 * n + 2] * u[edof[n] + 2];
-            }
-            cl += u[edof[ii] + a] * t;
+  for (int ii = 0; ii < 8; ii++) {
+#pragma omp parallel for schedule(static)
+    for (int64_t ch = 0; ch < nch; ch++) {
+      or_dtype ps = 0;
+      const int64_t t1 = (ch + 1) * OR_CHUNK < ne ? (ch + 1) * OR_CHUNK : ne;
+      for (int64_t t = ch * OR_CHUNK; t < t1; t++) {
+        const int64_t i = t / (nely * nelz) + 1, k = (t / nely) % nelz + 1, j = t % nely + 1;
+        int64_t edof[8];
+        for (int n = 0; n < 8; n++)
+          edof[n] = 3 * ((i + lnode[n][0]) * wy * wz + (k + lnode[n][2]) * wy + (j + lnode[n][1]));
+        const int64_t e = i * ewy * ewz + k * ewy + j;
+        or_stype cl = 0.0;
+        for (int a = 0; a < 3; a++) {
+          double tt = 0.0;
+          for (int n = 0; n < 8; n++) {
+            tt += KE[(ii * 3 + a) * 24 + 3 * n] * u[edof[n]];
+            tt += KE[(ii * 3 + a) * 24 + 3 * n + 1] * u[edof[n] + 1];
+            tt += KE[(ii * 3 + a) * 24 + 3 * n + 2] * u[edof[n] + 2];
           }
-          const or_dtype xe = x[e];
-          cc += cl * (Emin + xe * xe * xe * E_diff);
-          dcdx[e] += cl * (-3.0 * E_diff * xe * xe);
+          cl += u[edof[ii] + a] * tt;
         }
+        const or_dtype xe = x[e];
+        ps += cl * (Emin + xe * xe * xe * E_diff);
+        dcdx[e] += cl * (-3.0 * E_diff * xe * xe);
+      }
+      part[ch] = ps;
+    }
+    for (int64_t ch = 0; ch < nch; ch++) cc += part[ch];
+  }
+  free(part);
   *c = cc;
 }
 
@@ -1080,21 +1155,31 @@ float or_change(or_dtype *x, const or_dtype *xnew, int64_t nelem) {
 void or_oc(const or_dtype *x, or_dtype *xnew, const or_dtype *dv,
            const or_dtype *dc, int64_t nelem, or_dtype g) {
   or_dtype l1 = 0.0, l2 = 1e9, move = 0.2;
+  const int64_t nch = (nelem + OR_CHUNK - 1) / OR_CHUNK;
+  or_dtype *part = (or_dtype *)malloc(sizeof(or_dtype) * (nch > 0 ? nch : 1));
   while ((l2 - l1) / (l1 + l2) > 1e-6) {
     const or_dtype lmid = 0.5 * (l2 + l1);
     or_dtype gt = 0.0;
-    for (int64_t i = 0; i < nelem; i++) {
-      xnew[i] = OR_MAX(0.0, OR_MAX(x[i] - move,
-                                   OR_MIN(1.0, OR_MIN(x[i] + move,
-                                                      x[i] * sqrt(-dc[i] / (dv[i] * lmid))))));
-      gt += dv[i] * (xnew[i] - x[i]);
+#pragma omp parallel for schedule(static)
+    for (int64_t ch = 0; ch < nch; ch++) {
+      or_dtype ps = 0;
+      const int64_t i1 = (ch + 1) * OR_CHUNK < nelem ? (ch + 1) * OR_CHUNK : nelem;
+      for (int64_t i = ch * OR_CHUNK; i < i1; i++) {
+        xnew[i] = OR_MAX(0.0, OR_MAX(x[i] - move,
+                                     OR_MIN(1.0, OR_MIN(x[i] + move,
+                                                        x[i] * sqrt(-dc[i] / (dv[i] * lmid))))));
+        ps += dv[i] * (xnew[i] - x[i]);
+      }
+      part[ch] = ps;
     }
+    for (int64_t ch = 0; ch < nch; ch++) gt += part[ch];
     gt += g;
     if (gt > 0)
       l1 = lmid;
     else
       l2 = lmid;
   }
+  free(part);
 }
 
 /* ------------------------------------------------------------------------ */

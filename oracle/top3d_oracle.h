@@ -61,6 +61,8 @@ typedef struct {
   or_csr csr[OR_MAXL];
   int ncoarse;     /* unpadded DOFs of the coarsest level */
   double *chol;    /* dense lower Cholesky factor of the coarsest matrix */
+  int bw;          /* half bandwidth of the coarsest matrix */
+  double setup_s, loop_s; /* wall time of the last or_solve: set-up, CG loop */
 } or_solver;
 
 typedef struct {
@@ -119,6 +121,7 @@ void or_vcycle(const or_grid *g, or_solver *s, const or_dtype *x, int l,
 int or_solve(const or_grid *g, or_solver *s, or_dtype *x, int nswp,
              or_ctype tol, int *finalIter, float *finalRes, or_ctype *b,
              or_stype *u, int maxIter);
+void or_solver_times(const or_solver *s, double *setup_s, double *loop_s);
 
 void or_sensitivity(const or_grid *g, const or_dtype *x, const or_stype *u,
                     or_dtype *c, or_dtype *dcdx);

@@ -352,6 +352,81 @@ void top3d_solver_stats(const struct SolverDataBuffer *data, struct Top3dSolveSt
   if (s && out) *out = s->stats;
 }
 
+int top3d_time_kernel(struct SolverDataBuffer *data, int which, int reps, double *avg_ms,
+                      double *bytes) {
+  SolverState *s = solver_state(data);
+  if (!s) return top3d_last_error();
+  if (reps < 1 || which < 0 || which > 4) {
+    set_error(TOP3D_ERR_ARG, "top3d_time_kernel: which=%d reps=%d", which, reps);
+    return TOP3D_ERR_ARG;
+  }
+  const DevGrid &g = *s->g;
+  const Level &L0 = g.lv[0], &L1 = g.lv[1];
+  const double nn0 = (double)L0.interior_nodes(), ne0 = (double)L0.elements();
+  const double nn1 = (double)L1.interior_nodes();
+  // the FP32 buffers hold data only after a mixed solve; derive them here
+  launch_convert<float, double>(g.nelem, s->E64, s->E32);
+  for (int l = 0; l < 2; l++) {
+    launch_convert<float, double>(g.lv[l].ndof, s->invD64[l], s->invD32[l]);
+    launch_convert<float, double>(g.lv[l].ndof, s->v64[l][SolverState::kD],
+                                  s->v32[l][SolverState::kD]);
+  }
+  auto launch = [&](int rep) {
+    const bool odd = rep & 1;
+    switch (which) {
+      case 0: {
+        float *a = s->v32[0][SolverState::kZ], *b = s->t32[0];
+        launch_fine<float>(g, 1, s->E32, odd ? b : a, odd ? a : b, s->v32[0][SolverState::kD],
+                           s->invD32[0], (float)kOmega, s->bcsym);
+        break;
+      }
+      case 1: {
+        double *a = s->z, *b = s->t64[0];
+        launch_fine<double>(g, 1, s->E64, odd ? b : a, odd ? a : b, s->v64[0][SolverState::kD],
+                            s->invD64[0], kOmega, s->bcsym);
+        break;
+      }
+      case 2:
+        launch_fine<double>(g, 0, s->E64, s->p, s->q, nullptr, nullptr, 0.0, s->bcsym);
+        break;
+      case 3: {
+        float *a = s->v32[1][SolverState::kZ], *b = s->t32[1];
+        launch_level1<float>(g, 1, s->E32, odd ? b : a, odd ? a : b, s->v32[1][SolverState::kD],
+                             s->invD32[1], (float)kOmega, s->bcsym);
+        break;
+      }
+      default: {
+        double *a = s->v64[1][SolverState::kZ], *b = s->t64[1];
+        launch_level1<double>(g, 1, s->E64, odd ? b : a, odd ? a : b,
+                              s->v64[1][SolverState::kD], s->invD64[1], kOmega, s->bcsym);
+      }
+    }
+  };
+  launch(0);  // warm-up
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipEventRecord(e0, stream());
+  for (int r = 0; r < reps; r++) launch(r);
+  hipEventRecord(e1, stream());
+  if (!T3D_HIP(hipEventSynchronize(e1))) return TOP3D_ERR_HIP;
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (avg_ms) *avg_ms = ms / reps;
+  if (bytes) {
+    switch (which) {
+      case 0: *bytes = 48.0 * nn0 + 4.0 * ne0; break;   // u, b, invD in; u out; E
+      case 1: *bytes = 96.0 * nn0 + 8.0 * ne0; break;
+      case 2: *bytes = 48.0 * nn0 + 8.0 * ne0; break;   // p in, q out, E
+      case 3: *bytes = 48.0 * nn1 + 4.0 * ne0; break;   // level-1 vectors, fine E
+      default: *bytes = 96.0 * nn1 + 8.0 * ne0;
+    }
+  }
+  return TOP3D_OK;
+}
+
 // stencil_solvers.c:259-397
 void solveStateMG_halo(const struct gpuGrid *gpu_gc, DTYPE *x, const int nswp, const int nl,
                        const CTYPE tol, struct SolverDataBuffer *data, int *finalIter,

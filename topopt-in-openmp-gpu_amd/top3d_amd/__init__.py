@@ -146,6 +146,7 @@ SIGNATURES = {
     "top3d_compliance_forward": (c_int, [P, P, c_int, c_double, P, P, P, P]),
     "top3d_compliance_backward": (c_int, [P, c_double, P, P, c_double, P]),
     "top3d_dense_spd_inverse": (c_int, [P, c_int, c_int]),
+    "top3d_time_kernel": (c_int, [P, c_int, c_int, P, P]),
     "applyStateOperator_stencil_f32": (None, [P, P, P, P]),
     "applyStateOperatorSubspace_halo_f32": (None, [P, c_int, P, P, P]),
 }
@@ -400,6 +401,18 @@ class Grid:
         self.lib.top3d_solver_stats(ctypes.byref(self.data), ctypes.byref(s))
         return dict(iterations=s.iterations, relres=s.relres, setup_ms=s.setup_ms,
                     solve_ms=s.solve_ms, status=s.status)
+
+    KERNELS = {"fine_sweep_f32": 0, "fine_sweep_f64": 1, "fine_apply_f64": 2,
+               "level1_sweep_f32": 3, "level1_sweep_f64": 4}
+
+    def time_kernel(self, name, reps=20):
+        """(mean ms per launch, algorithmic bytes per launch) of a hot kernel."""
+        ms, by = c_double(), c_double()
+        rc = self.lib.top3d_time_kernel(ctypes.byref(self.data), self.KERNELS[name], reps,
+                                        ctypes.byref(ms), ctypes.byref(by))
+        if rc:
+            check("top3d_time_kernel")
+        return ms.value, by.value
 
 
 class DesignState:
