@@ -48,8 +48,13 @@ static double *mf_stencil(DevGrid &g, const double *E, int l) {
 static bool sync_ok() { return T3D_HIP(hipStreamSynchronize(stream())); }
 
 // Jacobi sweeps with the fused operator, ping-pong u <-> tmp, result in u.
+// The sweep kernels write the real nodes only; tmp starts as a copy of u so
+// that both buffers carry u's halo values, which the reference's sweeps leave
+// unchanged.
 template <typename F>
 static void sweeps(int64_t ndof, int nswp, double *u, double *tmp, F apply) {
+  if (nswp > 0)
+    hipMemcpyAsync(tmp, u, sizeof(double) * ndof, hipMemcpyDeviceToDevice, stream());
   double *cur = u, *oth = tmp;
   for (int s = 0; s < nswp; s++) {
     apply(cur, oth);
@@ -176,6 +181,7 @@ void applyStateOperator_stencil(const struct gpuNode *node, const DTYPE *x, cons
   if (!dx || !din || !dout) return;
   double *E = young64(*g, dx);
   launch_fine<double>(*g, kApplyMode, E, din, dout, nullptr, nullptr, 0.0, 0);
+  launch_zero_pad_rows(*g, dout);
   sync_ok();
 }
 
@@ -209,6 +215,7 @@ void applyStateOperatorSubspace_halo(const struct gpuNode *node, const int l, co
   double *E = young64(*g, dx);
   if (l == 0) {
     launch_fine<double>(*g, kApplyMode, E, din, dout, nullptr, nullptr, 0.0, 0);
+    launch_zero_pad_rows(*g, dout);
   } else if (l == 1) {
     launch_level1<double>(*g, kApplyMode, E, din, dout, nullptr, nullptr, 0.0, 0);
   } else {

@@ -1,0 +1,437 @@
+// Fine-level operator, element-centric, for gfx950.
+//
+// Reflection structure.  The element matrix KE commutes with the three
+// mirror reflections of the hexahedron.  In the per-component 8-point
+// Walsh-Hadamard basis of the element's nodal values,
+//   u^[q][c] = sum_p (-1)^{q.p} u[p][c]      (p, q in {0,1}^3, bit 0 = x),
+// KE becomes 8 independent 3x3 blocks: block s couples the components c with
+// q = s ^ (1 << c).  Blocks 1, 2, 4 contain a rigid translation (zero row and
+// column) and blocks 3, 5, 6 split into a 2x2 and a 1x1, so one element
+// product costs 72 + 45 + 21 + 72 operations (transform, blocks, modulus,
+// inverse transform) instead of 576 multiply-adds.  The host derives the
+// blocks from KE_0 and checks the zero pattern (hadamard_blocks()).
+//
+// Tiling.  A workgroup owns a tile of TY x TZ element columns in (y, z),
+// one thread per column, and marches along x over NX node planes.  Per
+// x-step each thread computes its element in the current slab, adds the
+// pending x+ face of the previous slab to the x- face of this one, keeps the
+// corner it owns (node (j+1, k+1)) and hands the other three corners to their
+// owners through LDS (9 values, one barrier).  Tiles overlap by one element
+// in every direction, so every owned node of a tile interior is complete
+// without atomics and the result is deterministic.
+#include <cmath>
+#include <vector>
+
+#include "t3d_internal.hpp"
+
+namespace t3d {
+
+enum { kModeApply = 0, kModeJacobi = 1, kModeResidual = 2 };
+
+// position bits (x, y, z) of the reference's local node n (stencil_utility.h:19-26)
+static constexpr int kPos[8][3] = {{0, 1, 0}, {1, 1, 0}, {1, 0, 0}, {0, 0, 0},
+                                   {0, 1, 1}, {1, 1, 1}, {1, 0, 1}, {0, 0, 1}};
+
+// Host: 3x3 blocks M[s][c][c'] of  T KE T^T / 64  (T: per-component WHT).
+// Returns false if KE does not have the reflection structure.
+bool hadamard_blocks(const double *KE, double M[8][3][3]) {
+  auto chi = [](int q, int n) {
+    const int d = (q & 1) * kPos[n][0] + ((q >> 1) & 1) * kPos[n][1] + ((q >> 2) & 1) * kPos[n][2];
+    return (d & 1) ? -1.0 : 1.0;
+  };
+  double Kt[24][24];
+  double scale = 0.0;
+  for (int qa = 0; qa < 8; qa++)
+    for (int ca = 0; ca < 3; ca++)
+      for (int qb = 0; qb < 8; qb++)
+        for (int cb = 0; cb < 3; cb++) {
+          double s = 0.0;
+          for (int a = 0; a < 8; a++)
+            for (int b = 0; b < 8; b++)
+              s += chi(qa, a) * KE[24 * (3 * a + ca) + 3 * b + cb] * chi(qb, b);
+          Kt[3 * qa + ca][3 * qb + cb] = s / 64.0;
+          scale = std::fmax(scale, std::fabs(s / 64.0));
+        }
+  const double tol = 1e-12 * scale;
+  auto sig = [](int idx) { return (idx / 3) ^ (1 << (idx % 3)); };
+  for (int a = 0; a < 24; a++)
+    for (int b = 0; b < 24; b++)
+      if (sig(a) != sig(b) && std::fabs(Kt[a][b]) > tol) return false;
+  for (int s = 0; s < 8; s++)
+    for (int c = 0; c < 3; c++)
+      for (int d = 0; d < 3; d++) {
+        const int qa = s ^ (1 << c), qb = s ^ (1 << d);
+        M[s][c][d] = Kt[3 * qa + c][3 * qb + d];
+      }
+  // zero pattern used by the kernel: translation rows/cols of the one-bit
+  // blocks, decoupled component of the two-bit blocks
+  for (int s = 1; s < 7; s++)
+    for (int c = 0; c < 3; c++)
+      for (int d = 0; d < 3; d++) {
+        const bool one_bit = (s & (s - 1)) == 0;
+        bool zero;
+        if (one_bit)
+          zero = (s == (1 << c)) || (s == (1 << d));
+        else
+          zero = (c != d) && (!((s >> c) & 1) || !((s >> d) & 1));
+        if (zero && std::fabs(M[s][c][d]) > tol) return false;
+        if (zero) M[s][c][d] = 0.0;
+      }
+  return true;
+}
+
+template <typename T>
+struct Blocks {
+  T m[8][3][3];
+};
+
+// y = E * KE u for one element; u, y indexed [px][py][pz][c]
+template <typename T>
+__device__ __forceinline__ void element_apply(const Blocks<T> &B, T E, const T (&u)[2][2][2][3],
+                                              T (&y)[2][2][2][3]) {
+  // forward transform, three butterfly stages; h[q][c], q = qx | qy<<1 | qz<<2
+  T h[8][3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    T a[2][2][2];
+#pragma unroll
+    for (int py = 0; py < 2; py++)
+#pragma unroll
+      for (int pz = 0; pz < 2; pz++) {
+        a[0][py][pz] = u[0][py][pz][c] + u[1][py][pz][c];
+        a[1][py][pz] = u[0][py][pz][c] - u[1][py][pz][c];
+      }
+    T b2[2][2][2];
+#pragma unroll
+    for (int qx = 0; qx < 2; qx++)
+#pragma unroll
+      for (int pz = 0; pz < 2; pz++) {
+        b2[qx][0][pz] = a[qx][0][pz] + a[qx][1][pz];
+        b2[qx][1][pz] = a[qx][0][pz] - a[qx][1][pz];
+      }
+#pragma unroll
+    for (int qx = 0; qx < 2; qx++)
+#pragma unroll
+      for (int qy = 0; qy < 2; qy++) {
+        h[qx | (qy << 1)][c] = b2[qx][qy][0] + b2[qx][qy][1];
+        h[qx | (qy << 1) | 4][c] = b2[qx][qy][0] - b2[qx][qy][1];
+      }
+  }
+  // block products, modulus, zero pattern of the reflection blocks
+  T g[8][3];
+#pragma unroll
+  for (int s = 0; s < 8; s++) {
+    T x[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) x[c] = h[s ^ (1 << c)][c];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      T acc = 0;
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        const bool one_bit = s != 0 && (s & (s - 1)) == 0;
+        const bool two_bit = s != 0 && s != 7 && !one_bit;
+        bool zero = false;
+        if (one_bit) zero = (s == (1 << c)) || (s == (1 << d));
+        if (two_bit) zero = (c != d) && (!((s >> c) & 1) || !((s >> d) & 1));
+        if (!zero) acc += B.m[s][c][d] * x[d];
+      }
+      g[s ^ (1 << c)][c] = E * acc;
+    }
+  }
+  // inverse transform
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    T a[2][2][2];  // [qx][qy][pz]
+#pragma unroll
+    for (int qx = 0; qx < 2; qx++)
+#pragma unroll
+      for (int qy = 0; qy < 2; qy++) {
+        const T lo = g[qx | (qy << 1)][c], hi = g[qx | (qy << 1) | 4][c];
+        a[qx][qy][0] = lo + hi;
+        a[qx][qy][1] = lo - hi;
+      }
+    T b2[2][2][2];  // [qx][py][pz]
+#pragma unroll
+    for (int qx = 0; qx < 2; qx++)
+#pragma unroll
+      for (int pz = 0; pz < 2; pz++) {
+        b2[qx][0][pz] = a[qx][0][pz] + a[qx][1][pz];
+        b2[qx][1][pz] = a[qx][0][pz] - a[qx][1][pz];
+      }
+#pragma unroll
+    for (int py = 0; py < 2; py++)
+#pragma unroll
+      for (int pz = 0; pz < 2; pz++) {
+        y[0][py][pz][c] = b2[0][py][pz] + b2[1][py][pz];
+        y[1][py][pz][c] = b2[0][py][pz] - b2[1][py][pz];
+      }
+  }
+}
+
+struct TileArgs {
+  int nelx, nely, nelz;  // fine elements
+  int wy, wz;            // node strides
+  int few, fez;          // element strides
+  int TY, TZ, NX;        // tile: element columns in y and z, node planes in x
+  int nty;               // tiles along y
+};
+
+template <typename T, int MODE, bool BCSYM>
+__global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_fine_tile(
+    TileArgs a, Blocks<T> B, const T *__restrict__ E, const T *__restrict__ in,
+    T *__restrict__ out, const T *__restrict__ b, const T *__restrict__ invD, T omega,
+    const uint8_t *__restrict__ fix) {
+  extern __shared__ unsigned char smem_raw[];
+  T *xch = reinterpret_cast<T *>(smem_raw);  // [2][nthreads][9]
+  const int nthr = a.TY * a.TZ;
+  const int t = threadIdx.x;
+  const bool active = t < nthr;
+  const int ty = t % a.TY, tz = t / a.TY;
+  const int by = blockIdx.y % a.nty, bz = blockIdx.y / a.nty;
+  const int j = by * (a.TY - 1) + ty;  // element column (halo index)
+  const int k = bz * (a.TZ - 1) + tz;
+  const int i0 = 1 + blockIdx.x * a.NX;
+  const int i1 = min(i0 + a.NX - 1, a.nelx + 1);  // last output plane
+  // the element exists in memory (0..nel+1) and may carry a non-zero modulus
+  const bool elem_ok = active && j <= a.nely && k <= a.nelz;
+  // owned node (j+1, k+1): complete if its other three elements are in the
+  // tile or are halo elements beyond the grid
+  const bool last_y = (ty == a.TY - 1), last_z = (tz == a.TZ - 1);
+  const bool own_ok = elem_ok && (!last_y || j + 1 == a.nely + 1) &&
+                      (!last_z || k + 1 == a.nelz + 1);
+  const int64_t sx = (int64_t)a.wy * a.wz;
+  const int64_t n00 = (int64_t)k * a.wy + j;  // node (j, k) within a plane
+  const int64_t noff[2][2] = {{0, a.wy}, {1, (int64_t)a.wy + 1}};  // [py][pz]
+  T u[2][2][2][3];
+  T pend[2][2][3];
+#pragma unroll
+  for (int py = 0; py < 2; py++)
+#pragma unroll
+    for (int pz = 0; pz < 2; pz++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) pend[py][pz][c] = 0;
+
+  auto load_plane = [&](int px, int i) {
+#pragma unroll
+    for (int py = 0; py < 2; py++)
+#pragma unroll
+      for (int pz = 0; pz < 2; pz++) {
+        T v0 = 0, v1 = 0, v2 = 0;
+        if (elem_ok && i <= a.nelx + 2) {
+          const int64_t nd = i * sx + n00 + noff[py][pz];
+          v0 = in[3 * nd];
+          v1 = in[3 * nd + 1];
+          v2 = in[3 * nd + 2];
+          if (BCSYM) {
+            const uint8_t m = fix[nd];
+            if (m & 1) v0 = 0;
+            if (m & 2) v1 = 0;
+            if (m & 4) v2 = 0;
+          }
+        }
+        u[px][py][pz][0] = v0;
+        u[px][py][pz][1] = v1;
+        u[px][py][pz][2] = v2;
+      }
+  };
+
+  load_plane(1, i0 - 1);
+  for (int i = i0 - 1; i <= i1; i++) {  // element slab i: planes i, i+1
+#pragma unroll
+    for (int py = 0; py < 2; py++)
+#pragma unroll
+      for (int pz = 0; pz < 2; pz++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) u[0][py][pz][c] = u[1][py][pz][c];
+    load_plane(1, i + 1);
+    T Ee = 0;
+    if (elem_ok && i <= a.nelx + 1) Ee = E[((int64_t)i * a.fez + k) * a.few + j];
+    T y[2][2][2][3];
+    element_apply<T>(B, Ee, u, y);
+    if (i >= i0) {
+      // plane i: x- face of slab i plus the pending x+ face of slab i-1
+      T tot[2][2][3];
+#pragma unroll
+      for (int py = 0; py < 2; py++)
+#pragma unroll
+        for (int pz = 0; pz < 2; pz++)
+#pragma unroll
+          for (int c = 0; c < 3; c++) tot[py][pz][c] = y[0][py][pz][c] + pend[py][pz][c];
+      T *buf = xch + (size_t)(i & 1) * nthr * 9;
+      if (active) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          buf[9 * t + c] = tot[0][1][c];      // node (j, k+1)  -> thread (ty-1, tz)
+          buf[9 * t + 3 + c] = tot[1][0][c];  // node (j+1, k)  -> thread (ty, tz-1)
+          buf[9 * t + 6 + c] = tot[0][0][c];  // node (j, k)    -> thread (ty-1, tz-1)
+        }
+      }
+      __syncthreads();
+      if (own_ok) {
+        T kin[3] = {tot[1][1][0], tot[1][1][1], tot[1][1][2]};
+        if (!last_y) {
+#pragma unroll
+          for (int c = 0; c < 3; c++) kin[c] += buf[9 * (t + 1) + c];
+        }
+        if (!last_z) {
+#pragma unroll
+          for (int c = 0; c < 3; c++) kin[c] += buf[9 * (t + a.TY) + 3 + c];
+        }
+        if (!last_y && !last_z) {
+#pragma unroll
+          for (int c = 0; c < 3; c++) kin[c] += buf[9 * (t + a.TY + 1) + 6 + c];
+        }
+        const int jn = j + 1, kn = k + 1;
+        if (jn >= 1 && kn >= 1) {  // owned node is a real node (i is real)
+          const int64_t nd = i * sx + (int64_t)kn * a.wy + jn;
+          const uint8_t m = fix[nd];
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            const T uc = u[0][1][1][c];
+            T kv = kin[c];
+            T o;
+            if (BCSYM) {  // the loaded value is masked; the BC row needs in
+              if ((m >> c) & 1) kv = in[3 * nd + c];
+            } else {
+              if ((m >> c) & 1) kv = uc;
+            }
+            if (MODE == kModeApply) {
+              o = kv;
+            } else if (MODE == kModeJacobi) {
+              const T uo = BCSYM ? in[3 * nd + c] : uc;
+              o = uo + omega * invD[3 * nd + c] * (b[3 * nd + c] - kv);
+            } else {
+              o = b[3 * nd + c] - kv;
+            }
+            out[3 * nd + c] = o;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int py = 0; py < 2; py++)
+#pragma unroll
+      for (int pz = 0; pz < 2; pz++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) pend[py][pz][c] = y[1][py][pz][c];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+static bool g_blocks_ok = false;
+static Blocks<double> g_b64;
+static Blocks<float> g_b32;
+
+static void init_blocks(const DevGrid &g) {
+  if (g_blocks_ok) return;
+  std::vector<double> KE(576);
+  element_matrices(0.3, 0, KE.data());
+  double M[8][3][3];
+  if (!hadamard_blocks(KE.data(), M)) {
+    set_error(TOP3D_ERR_ARG, "element matrix lacks the reflection structure");
+    return;
+  }
+  for (int s = 0; s < 8; s++)
+    for (int c = 0; c < 3; c++)
+      for (int d = 0; d < 3; d++) {
+        g_b64.m[s][c][d] = M[s][c][d];
+        g_b32.m[s][c][d] = (float)M[s][c][d];
+      }
+  g_blocks_ok = true;
+  (void)g;
+}
+
+struct TileShape {
+  int TY, TZ, NX, nty, ntz, ntx;
+};
+
+static TileShape tile_shape(const DevGrid &g, int elem_bytes) {
+  const Level &L = g.lv[0];
+  TileShape s;
+  // LDS: two exchange buffers of 9 values per thread, kept under 60 KiB
+  const int lds_threads = 60 * 1024 / (18 * elem_bytes);
+  const int max_threads = std::min(elem_bytes == 4 ? 1024 : 512, lds_threads);
+  const int ycap = elem_bytes == 4 ? 129 : 65;
+  s.TY = std::max(2, std::min(L.nely + 1, ycap));
+  s.TZ = std::max(2, std::min(max_threads / s.TY, 16));
+  // a tile owns TY-1 node rows, plus the last row of the grid (its upper
+  // neighbour element is halo)
+  auto tiles = [](int nodes, int T) { return nodes <= T ? 1 : (nodes - T + T - 2) / (T - 1) + 1; };
+  s.nty = tiles(L.ny, s.TY);
+  s.ntz = tiles(L.nz, s.TZ);
+  // about two workgroups per CU (256 CUs); longer x-marches waste less
+  const int64_t yz = (int64_t)s.nty * s.ntz;
+  int ntx = (int)std::max<int64_t>(1, (512 + yz - 1) / yz);
+  ntx = std::min(ntx, L.nx);
+  s.NX = (L.nx + ntx - 1) / ntx;
+  if (s.NX < 4) s.NX = std::min(4, L.nx);
+  s.ntx = (L.nx + s.NX - 1) / s.NX;
+  return s;
+}
+
+template <typename T>
+void launch_fine_tile(const DevGrid &g, int mode, const T *E, const T *in, T *out,
+                      const T *b, const T *invD, T omega, int bcsym) {
+  init_blocks(g);
+  const Level &L = g.lv[0];
+  const TileShape s = tile_shape(g, (int)sizeof(T));
+  TileArgs a;
+  a.nelx = L.nelx;
+  a.nely = L.nely;
+  a.nelz = L.nelz;
+  a.wy = L.wy;
+  a.wz = L.wz;
+  a.few = g.ewy;
+  a.fez = g.ewz;
+  a.TY = s.TY;
+  a.TZ = s.TZ;
+  a.NX = s.NX;
+  a.nty = s.nty;
+  const int nthr = s.TY * s.TZ;
+  const int bs = (nthr + 63) / 64 * 64;
+  const size_t shm = sizeof(T) * 2 * 9 * (size_t)nthr;
+  dim3 grid(s.ntx, s.nty * s.ntz);
+  Blocks<T> B;
+  if constexpr (sizeof(T) == 8) B = g_b64; else B = g_b32;
+#define T3D_LAUNCH(M, S)                                                                   \
+  hipLaunchKernelGGL((k_fine_tile<T, M, S>), grid, dim3(bs), shm, stream(), a, B, E, in, \
+                     out, b, invD, omega, g.fixmask[0])
+  if (bcsym) {
+    if (mode == kModeApply) T3D_LAUNCH(kModeApply, true);
+    else if (mode == kModeJacobi) T3D_LAUNCH(kModeJacobi, true);
+    else T3D_LAUNCH(kModeResidual, true);
+  } else {
+    if (mode == kModeApply) T3D_LAUNCH(kModeApply, false);
+    else if (mode == kModeJacobi) T3D_LAUNCH(kModeJacobi, false);
+    else T3D_LAUNCH(kModeResidual, false);
+  }
+#undef T3D_LAUNCH
+}
+
+// rows j > nely+1 of every (i, k) set to zero, as the reference's operator
+// does for its padded y nodes (stencil_methods.c:407-418)
+__global__ void k_zero_pad_rows(int64_t nrows, int wy, int first, double *__restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int npad = wy - first;
+  if (t >= nrows * npad) return;
+  const int64_t node = (t / npad) * wy + first + t % npad;
+  out[3 * node] = out[3 * node + 1] = out[3 * node + 2] = 0.0;
+}
+
+void launch_zero_pad_rows(const DevGrid &g, double *out) {
+  const Level &L = g.lv[0];
+  const int first = L.nely + 2;
+  if (first >= L.wy) return;
+  const int64_t n = (int64_t)L.wx * L.wz * (L.wy - first);
+  hipLaunchKernelGGL(k_zero_pad_rows, dim3((n + 255) / 256), dim3(256), 0, stream(),
+                     (int64_t)L.wx * L.wz, L.wy, first, out);
+}
+
+template void launch_fine_tile<double>(const DevGrid &, int, const double *, const double *,
+                                       double *, const double *, const double *, double, int);
+template void launch_fine_tile<float>(const DevGrid &, int, const float *, const float *,
+                                      float *, const float *, const float *, float, int);
+
+}  // namespace t3d

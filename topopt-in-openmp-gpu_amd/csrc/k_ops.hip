@@ -10,6 +10,9 @@
 //   kApply    out = K in                     (fixed DOFs: out = in)
 //   kJacobi   out = in + w invD (b - K in)   (interior nodes; else out = in)
 //   kResidual out = b - K in
+#include <cstdlib>
+#include <string>
+
 #include "t3d_internal.hpp"
 
 namespace t3d {
@@ -146,9 +149,23 @@ __global__ __launch_bounds__(256) void k_fine(
   }
 }
 
+// node-centric variant kept for cross-checking (TOP3D_FINE_KERNEL=node)
+static bool use_node_kernel() {
+  static int v = -1;
+  if (v < 0) {
+    const char *e = std::getenv("TOP3D_FINE_KERNEL");
+    v = (e && std::string(e) == "node") ? 1 : 0;
+  }
+  return v == 1;
+}
+
 template <typename T>
 void launch_fine(const DevGrid &g, int mode, const T *E, const T *in, T *out,
                  const T *b, const T *invD, T omega, int bcsym) {
+  if (!use_node_kernel()) {
+    launch_fine_tile<T>(g, mode, E, in, out, b, invD, omega, bcsym);
+    return;
+  }
   const Level &L = g.lv[0];
   const LevelArgs a = g.args(0);
   const T *KE = nullptr;

@@ -91,6 +91,11 @@ void launch_young_float_input(const DevGrid &g, const float *x, T *E);
 template <typename T>
 void launch_fine(const DevGrid &g, int mode, const T *E, const T *in, T *out,
                  const T *b, const T *invD, T omega, int bcsym);
+template <typename T>
+void launch_fine_tile(const DevGrid &g, int mode, const T *E, const T *in, T *out,
+                      const T *b, const T *invD, T omega, int bcsym);
+// zero the padded y rows (j > nely+1) of a level-0 vector
+void launch_zero_pad_rows(const DevGrid &g, double *out);
 // level-1 matrix-free operator with fine moduli, same modes
 template <typename T>
 void launch_level1(const DevGrid &g, int mode, const T *E, const T *in, T *out,
