@@ -6,6 +6,8 @@ compare against oracle runs too long to repeat inside a test:
   c1_fp64.json / c1_asship.json   48x24x24, 3 levels, 5 design iterations
   c2_fp64.json                    128x64x64, 4 levels, 20 design iterations
   c1_xphys_fp64.npz               final filtered densities of c1 (interior)
+  c2_xphys_fp64.npz               final filtered densities of c2 (interior),
+                                  quantized to uint16 steps of 1/65535
 Run from the repository root: python tests/golden/make_golden.py [c1|c2]
 """
 import json
@@ -48,6 +50,10 @@ def run(name):
         if name == "c1" and typing == "fp64":
             np.savez_compressed(os.path.join(HERE, "c1_xphys_fp64.npz"),
                                 xphys=interior(xphys, c["nelx"], c["nely"], c["nelz"]))
+        if name == "c2" and typing == "fp64":
+            xi = interior(xphys, c["nelx"], c["nely"], c["nelz"])
+            q = np.rint(np.clip(xi, 0.0, 1.0) * 65535.0).astype(np.uint16)
+            np.savez_compressed(os.path.join(HERE, "c2_xphys_fp64.npz"), xphys_u16=q)
         print(name, typing, f"{time.time() - t0:.1f}s", flush=True)
 
 

@@ -109,8 +109,8 @@ __global__ void k_zero_noninterior(int64_t nodes, int nx, int ny, int nz, int wy
 
 // Coarse direct solver object behind initializeCholmod/factorize/solve.
 struct CoarseObj {
-  int l = 0, n = 0, bw = 0;
-  double *A = nullptr, *work = nullptr, *cw = nullptr;
+  int l = 0;
+  CoarseFactor f;
 };
 
 static void upload_csr(const struct CSRMatrix &M, int **ro, int **ci, double **va) {
@@ -521,12 +521,12 @@ void initializeCholmod(const struct gridContext gc, const int l,
   if (!g) return;
   CoarseObj *c = new CoarseObj();
   c->l = l;
-  c->n = M.nrows;
-  c->bw = 3 * (g->lv[l].ny * g->lv[l].nz + g->lv[l].ny + 1) + 2;
-  const int nblk = (c->n + 63) / 64;
-  c->A = (double *)dalloc(sizeof(double) * (size_t)c->n * c->n);
-  c->work = (double *)dalloc(sizeof(double) * ((size_t)c->n * c->n + (size_t)nblk * 4096 + 8));
-  c->cw = (double *)dalloc(sizeof(double) * c->n);
+  const int bw = 3 * (g->lv[l].ny * g->lv[l].nz + g->lv[l].ny + 1) + 2;
+  if (!c->f.allocate(M.nrows, bw, false)) {
+    c->f.release();
+    delete c;
+    return;
+  }
   solverData->factoredMatrix = c;
 }
 
@@ -534,11 +534,9 @@ void finishCholmod(const struct gridContext gc, const int l, struct CoarseSolver
                    const struct CSRMatrix M, const int verbose) {
   CoarseObj *c = (CoarseObj *)solverData->factoredMatrix;
   if (!c) return;
-  if (verbose == 1) std::printf("coarse solver: n=%d, half bandwidth %d\n", c->n, c->bw);
+  if (verbose == 1) std::printf("coarse solver: n=%d, half bandwidth %d\n", c->f.n, c->f.bw);
   hipStreamSynchronize(stream());
-  dfree(c->A);
-  dfree(c->work);
-  dfree(c->cw);
+  c->f.release();
   delete c;
   solverData->factoredMatrix = nullptr;
   (void)gc; (void)l; (void)M;
@@ -569,11 +567,11 @@ void factorizeSubspaceMatrix(const struct gridContext gc, const int l,
   int *ro, *ci;
   double *va;
   upload_csr(M, &ro, &ci, &va);
-  launch_fill<double>((int64_t)c->n * c->n, c->A, 0.0);
+  launch_fill<double>((int64_t)c->f.n * c->f.n, c->f.A, 0.0);
   const int bs = 256;
-  hipLaunchKernelGGL(k_csr_lower_to_dense, dim3((c->n + bs - 1) / bs), dim3(bs), 0, stream(),
-                     (int64_t)c->n, ro, ci, va, c->A);
-  dense_cholesky_inverse_banded(c->A, c->n, c->bw, c->work);
+  hipLaunchKernelGGL(k_csr_lower_to_dense, dim3((c->f.n + bs - 1) / bs), dim3(bs), 0, stream(),
+                     (int64_t)c->f.n, ro, ci, va, c->f.A);
+  c->f.factor();
   dfree(ro);
   dfree(ci);
   dfree(va);
@@ -589,7 +587,7 @@ void solveSubspaceMatrix(const struct gridContext gc, const int l,
   const double *din = (const double *)dptr(in, "in");
   double *dout = (double *)dptr(out, "out");
   if (!din || !dout) return;
-  launch_symv_coarse<double>(*g, l, c->A, c->n, din, dout, c->cw);
+  launch_coarse_apply<double>(*g, l, c->f, din, dout);
   sync_ok();
 }
 

@@ -185,55 +185,4 @@ void launch_stencil_to_dense(const DevGrid &g, int l, const double *S, double *A
                      stream(), L.nx, L.ny, L.nz, S, A, (int64_t)n);
 }
 
-// ---------------------------------------------------------------------------
-// Coarsest solve z = A^{-1} r with the explicit inverse (row-major, n x n):
-// gather the interior DOFs, one wavefront per row, scatter back to the halo
-// layout (the halo of z stays zero).
-template <typename T>
-__global__ void k_gather_interior(int nx, int ny, int nz, int wy, int wz,
-                                  const T *__restrict__ in, T *__restrict__ out) {
-  const int64_t nint = (int64_t)nx * ny * nz;
-  const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (d >= 3 * nint) return;
-  const int64_t p = d / 3;
-  const int j = (int)(p % ny), k = (int)((p / ny) % nz), i = (int)(p / ((int64_t)ny * nz));
-  out[d] = in[3 * (((int64_t)(i + 1) * wz + (k + 1)) * wy + (j + 1)) + d % 3];
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_gemv_scatter(int nx, int ny, int nz, int wy, int wz,
-                                                      const T *__restrict__ A, int n,
-                                                      const T *__restrict__ x,
-                                                      T *__restrict__ out) {
-  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (row >= n) return;
-  const T *a = A + (int64_t)row * n;
-  double s = 0.0;
-  for (int c = lane; c < n; c += 64) s += (double)a[c] * (double)x[c];
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-  if (lane == 0) {
-    const int64_t p = row / 3;
-    const int j = (int)(p % ny), k = (int)((p / ny) % nz), i = (int)(p / ((int64_t)ny * nz));
-    out[3 * (((int64_t)(i + 1) * wz + (k + 1)) * wy + (j + 1)) + row % 3] = (T)s;
-  }
-}
-
-template <typename T>
-void launch_symv_coarse(const DevGrid &g, int l, const T *Ainv, int n, const T *in, T *out,
-                        T *work) {
-  const Level &L = g.lv[l];
-  const int bs = 256;
-  hipLaunchKernelGGL((k_gather_interior<T>), dim3((n + bs - 1) / bs), dim3(bs), 0, stream(),
-                     L.nx, L.ny, L.nz, L.wy, L.wz, in, work);
-  hipLaunchKernelGGL((k_gemv_scatter<T>), dim3((n + 3) / 4), dim3(256), 0, stream(), L.nx,
-                     L.ny, L.nz, L.wy, L.wz, Ainv, n, work, out);
-}
-
-template void launch_symv_coarse<double>(const DevGrid &, int, const double *, int,
-                                         const double *, double *, double *);
-template void launch_symv_coarse<float>(const DevGrid &, int, const float *, int,
-                                        const float *, float *, float *);
-
 }  // namespace t3d

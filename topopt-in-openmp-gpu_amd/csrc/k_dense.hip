@@ -1,126 +1,355 @@
 // Dense FP64 direct solver of the coarsest level (replaces CHOLMOD,
 // stencil_assembly.c:366-503).
 //
-// The coarsest matrix (n = 3 * nodes, <= ~4k) is banded: with the x-slowest
-// node order its half bandwidth is 3*(ny*nz + ny + 1) + 2, and Cholesky
-// keeps that band.  Per design iteration:
-//   1. banded right-looking blocked Cholesky A = L L^T (64-wide blocks): the
-//      diagonal block on one workgroup, the panel by row-parallel
-//      substitution, the trailing band update on FP64 MFMA
-//      (v_mfma_f64_16x16x4f64), the only dense contraction of the solver;
-//   2. inverses of the 64x64 diagonal blocks of L;
-//   3. the explicit inverse W = A^{-1} from W = I by a forward and a backward
-//      banded solve with n right-hand sides, one workgroup per 16-column
-//      strip, every triangular step turned into small GEMMs.
-// Each V-cycle then applies A^{-1} as one parallel GEMV.
+// The coarsest matrix (n = 3 * interior nodes, 4131 at 16x8x8 elements) is
+// banded: with the x-slowest node order its half bandwidth is
+// 3*(ny*nz + ny + 1) + 2, and Cholesky keeps that band.  Per design iteration:
+//   1. banded right-looking Cholesky A = L L^T over 64x64 tiles, one launch
+//      per block column b: each workgroup owns one tile (I, J) of the trailing
+//      band, forms the panel tiles Q_I = A_Ib L_bb^{-T} and Q_J on FP64 MFMA
+//      (v_mfma_f64_16x16x4f64) and applies C_IJ -= Q_I Q_J^T; the diagonal
+//      workgroups write the panel of L, and the workgroup of tile
+//      (b+1, b+1) factors it in LDS right after its update (Cholesky, then
+//      the inverse of the 64x64 factor);
+//   2. W = L^{-1} (dense lower triangle), one workgroup per 16-column strip
+//      marching down the block rows with the last band of block rows in LDS,
+//      every step a band GEMM plus a product with the inverse diagonal block,
+//      both on MFMA; W^T is written alongside.
+// A V-cycle applies A^{-1} = W^T W as two triangular GEMVs.
 #include <algorithm>
 
 #include "t3d_internal.hpp"
 
 namespace t3d {
 
-constexpr int kNB = 64;
+constexpr int kNB = 64;         // tile size
+constexpr int kLd = kNB + 1;    // LDS row stride of a 64x64 tile
+constexpr int kStrip = 16;      // columns per workgroup of the W = L^{-1} kernel
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
-// ---- 1a. Cholesky of the diagonal block (left-looking, in LDS) -------------
-__global__ __launch_bounds__(64) void k_potrf_diag(double *__restrict__ A, int64_t n,
-                                                   int k0, int kb, int *__restrict__ info) {
-  __shared__ double L[kNB][kNB + 1];
-  const int t = threadIdx.x;
-  for (int r = 0; r < kb; r++)
-    if (t < kb) L[r][t] = (t <= r) ? A[(k0 + r) * n + k0 + t] : 0.0;
-  __syncthreads();
-  for (int j = 0; j < kb; j++) {
-    if (t == 0) {
-      double d = L[j][j];
-      for (int m = 0; m < j; m++) d -= L[j][m] * L[j][m];
-      if (!(d > 0.0)) {
-        atomicExch(info, 1);
-        d = 1.0;
-      }
-      L[j][j] = sqrt(d);
-    }
-    __syncthreads();
-    if (t > j && t < kb) {
-      double v = L[t][j];
-      for (int m = 0; m < j; m++) v -= L[t][m] * L[j][m];
-      L[t][j] = v / L[j][j];
-    }
-    __syncthreads();
-  }
-  for (int r = 0; r < kb; r++)
-    if (t <= r) A[(k0 + r) * n + k0 + t] = L[r][t];
+// MFMA operand layout (v_mfma_f64_16x16x4f64, measured on gfx950): lane l
+// feeds A[l%16][l/16] and B[l/16][l%16]; accumulator register v of lane l
+// holds D[l/16 + 4v][l%16].
+__device__ inline double4_t mfma4(double a, double b, double4_t c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-// ---- 1b. panel: A[i, k0:k0+kb] <- A[i, k0:k0+kb] L_kk^{-T}, rows in band --
-__global__ __launch_bounds__(64) void k_trsm_panel(double *__restrict__ A, int64_t n, int k0,
-                                                   int kb, int r0, int r1) {
-  __shared__ double L[kNB][kNB + 1];
-  const int t = threadIdx.x;
-  for (int r = 0; r < kb; r++)
-    if (t < kb) L[r][t] = (t <= r) ? A[(k0 + r) * n + k0 + t] : 0.0;
-  __syncthreads();
-  const int i = r0 + blockIdx.x * 64 + t;
-  if (i >= r1) return;
-  double x[kNB];
-  double *row = A + (int64_t)i * n + k0;
-#pragma unroll
-  for (int j = 0; j < kNB; j++) x[j] = j < kb ? row[j] : 0.0;
-#pragma unroll
-  for (int j = 0; j < kNB; j++) {
-    if (j < kb) {
-      double v = x[j];
-#pragma unroll
-      for (int m = 0; m < j; m++) v -= x[m] * L[j][m];
-      x[j] = v / L[j][j];
-    }
-  }
-#pragma unroll
-  for (int j = 0; j < kNB; j++)
-    if (j < kb) row[j] = x[j];
-}
-
-// ---- 1c. trailing update C_IJ -= P_I P_J^T on FP64 MFMA --------------------
-// One workgroup (4 waves) per 64x64 tile of the trailing band, each wave a
-// 32x32 quadrant as 2x2 blocks of v_mfma_f64_16x16x4f64.  MFMA operand
-// layout (16x16x4, f64, measured on gfx950): lane l feeds A[l%16][l/16] and
-// B[l/16][l%16]; accumulator register v of lane l holds D[l/16 + 4v][l%16].
-__global__ __launch_bounds__(256) void k_band_update(double *__restrict__ A, int64_t n, int k0,
-                                                     int kb, int t0, int ntiles, int bwt) {
-  __shared__ double Pi[kNB][kNB + 1];
-  __shared__ double Pj[kNB][kNB + 1];
-  // tile enumeration: row tile I = t0 + a, col tile J = I - d, d in [0, bwt]
-  const int a = blockIdx.x / (bwt + 1), d = blockIdx.x % (bwt + 1);
-  const int I = t0 + a, J = I - d;
-  if (a >= ntiles || J < t0) return;
-  const int i0 = I * kNB, j0 = J * kNB;
-  const int tid = threadIdx.x;
-  for (int e = tid; e < kNB * kNB; e += 256) {
-    const int r = e / kNB, c = e % kNB;
-    Pi[r][c] = (i0 + r < n && c < kb) ? A[(int64_t)(i0 + r) * n + k0 + c] : 0.0;
-    Pj[r][c] = (j0 + r < n && c < kb) ? A[(int64_t)(j0 + r) * n + k0 + c] : 0.0;
-  }
-  __syncthreads();
-  const int w = tid >> 6, lane = tid & 63;
-  const int qr = (w >> 1) * 32, qc = (w & 1) * 32;
-  double4_t acc[2][2];
-#pragma unroll
-  for (int x = 0; x < 2; x++)
-#pragma unroll
-    for (int y = 0; y < 2; y++) acc[x][y] = double4_t{0.0, 0.0, 0.0, 0.0};
+// acc += X Y^T on the wave's 32x32 quadrant (rows qr.., cols qc..) of a
+// 64x64 product; X, Y are 64x64 tiles in LDS with row stride kLd.
+__device__ inline void mm_nt_tile(const double *X, const double *Y, int qr, int qc, int lane,
+                                  double4_t acc[2][2]) {
   const int li = lane & 15, lk = lane >> 4;
   for (int k = 0; k < kNB; k += 4) {
     double av[2], bv[2];
 #pragma unroll
-    for (int x = 0; x < 2; x++) av[x] = Pi[qr + 16 * x + li][k + lk];
+    for (int x = 0; x < 2; x++) av[x] = X[(qr + 16 * x + li) * kLd + k + lk];
 #pragma unroll
-    for (int y = 0; y < 2; y++) bv[y] = Pj[qc + 16 * y + li][k + lk];
+    for (int y = 0; y < 2; y++) bv[y] = Y[(qc + 16 * y + li) * kLd + k + lk];
 #pragma unroll
     for (int x = 0; x < 2; x++)
 #pragma unroll
-      for (int y = 0; y < 2; y++)
-        acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x], bv[y], acc[x][y], 0, 0, 0);
+      for (int y = 0; y < 2; y++) acc[x][y] = mfma4(av[x], bv[y], acc[x][y]);
+  }
+}
+
+__device__ inline void zero_acc(double4_t acc[2][2]) {
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++) acc[x][y] = double4_t{0.0, 0.0, 0.0, 0.0};
+}
+
+// Store the wave's quadrant of an accumulator into a 64x64 LDS tile.
+__device__ inline void store_acc(double *X, const double4_t acc[2][2], int qr, int qc, int lane) {
+  const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) X[(qr + 16 * x + lk + 4 * v) * kLd + qc + 16 * y + li] = acc[x][y][v];
+}
+
+// ---------------------------------------------------------------------------
+// Cholesky and inverse of one 64x64 SPD tile in LDS, 256 threads.
+// In: T (lower triangle; rows/cols beyond the matrix padded with identity).
+// Out: T = L (zeros above the diagonal), X = L^{-1}.  A non-positive pivot
+// sets *info (and is replaced by 1 so that the sweep finishes).
+__device__ void factor_tile(double *T, double *X, double *Ys, double *dsq, int *info) {
+  const int t = threadIdx.x;
+  const int r = t >> 2, s = t & 3;  // thread owns (r, 4q+s), q = 0..15
+  double v[16];
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const int c = 4 * q + s;
+    v[q] = (c <= r) ? T[r * kLd + c] : 0.0;
+  }
+  // right-looking with deferred column scaling: a_rc -= a_rj a_cj / a_jj;
+  // column j+1 is final after step j and is published to T by its owners
+#pragma unroll
+  for (int j = 0; j < kNB; j++) {
+    double ajj = T[j * kLd + j];
+    if (!(ajj > 0.0)) {
+      if (t == 0) info[0] = 1;
+      ajj = 1.0;
+    }
+    const double f = T[r * kLd + j] / ajj;
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int c = 4 * q + s;
+      if (c > j && c <= r) v[q] -= f * T[c * kLd + j];
+    }
+    if (j + 1 < kNB && s == ((j + 1) & 3) && r >= j + 1) T[r * kLd + j + 1] = v[(j + 1) >> 2];
+    __syncthreads();
+  }
+  if (t < kNB) {
+    const double d = T[t * kLd + t];
+    dsq[t] = sqrt(d > 0.0 ? d : 1.0);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const int c = 4 * q + s;
+    T[r * kLd + c] = (c <= r) ? v[q] / dsq[c] : 0.0;
+  }
+  for (int e = t; e < kNB * kNB; e += 256) X[(e >> 6) * kLd + (e & 63)] = 0.0;
+  __syncthreads();
+  // inverse: the four 16x16 diagonal blocks by substitution (one wave each)
+  const int w = t >> 6, lane = t & 63;
+  if (lane < 16) {
+    const int b0 = 16 * w, c = lane;
+    double x[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      double acc = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+      for (int m = 0; m < i; m++) acc -= T[(b0 + i) * kLd + b0 + m] * x[m];
+      x[i] = acc / T[(b0 + i) * kLd + b0 + i];
+      X[(b0 + i) * kLd + b0 + c] = x[i];
+    }
+  }
+  __syncthreads();
+  // off-diagonal blocks by distance d: X_IJ = -X_II sum_{K=J}^{I-1} L_IK X_KJ
+  const int rr = t >> 4, cc = t & 15;
+  for (int d = 1; d < 4; d++) {
+    const int ntile = 4 - d;
+    for (int a = 0; a < ntile; a++) {
+      const int I = d + a, J = a;
+      double y = 0.0;
+      for (int K = J; K < I; K++)
+#pragma unroll
+        for (int m = 0; m < 16; m++)
+          y += T[(16 * I + rr) * kLd + 16 * K + m] * X[(16 * K + m) * kLd + 16 * J + cc];
+      Ys[a * 256 + t] = y;
+    }
+    __syncthreads();
+    for (int a = 0; a < ntile; a++) {
+      const int I = d + a, J = a;
+      double xv = 0.0;
+#pragma unroll
+      for (int m = 0; m < 16; m++) xv -= X[(16 * I + rr) * kLd + 16 * I + m] * Ys[a * 256 + m * 16 + cc];
+      X[(16 * I + rr) * kLd + 16 * J + cc] = xv;
+    }
+    __syncthreads();
+  }
+}
+
+// Write the factored diagonal tile (blk, blk): L to Lf, L^{-1} to Dinv.
+__device__ inline void store_diag(const double *T, const double *X, double *__restrict__ Lf,
+                                  double *__restrict__ Dinv, int n, int blk) {
+  const int k0 = blk * kNB;
+  for (int e = threadIdx.x; e < kNB * kNB; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    if (k0 + r < n && k0 + c < n) Lf[(int64_t)(k0 + r) * n + k0 + c] = T[r * kLd + c];
+    Dinv[(int64_t)blk * kNB * kNB + e] = X[r * kLd + c];
+  }
+}
+
+// One block column b of the banded Cholesky (b = -1: factor tile (0, 0)).
+// Workgroup p handles trailing tile (I, J), b+1 <= J <= I <= t1, enumerated
+// row by row.  A keeps the Schur-complement updates; Lf receives L.
+__global__ __launch_bounds__(256) void k_chol_step(double *__restrict__ A, double *__restrict__ Lf,
+                                                   double *__restrict__ Dinv, int n, int b,
+                                                   int *__restrict__ info) {
+  __shared__ double PI[kNB * kLd];
+  __shared__ double PJ[kNB * kLd];
+  __shared__ double DV[kNB * kLd];
+  __shared__ double Ys[3 * 256];
+  __shared__ double dsq[kNB];
+  const int tid = threadIdx.x;
+  if (b < 0) {
+    for (int e = tid; e < kNB * kNB; e += 256) {
+      const int r = e >> 6, c = e & 63;
+      DV[r * kLd + c] = (r < n && c < n) ? A[(int64_t)r * n + c] : (r == c ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    factor_tile(DV, PJ, Ys, dsq, info);
+    store_diag(DV, PJ, Lf, Dinv, n, 0);
+    return;
+  }
+  int p = blockIdx.x, I = b + 1;
+  while (p >= I - b) {
+    p -= I - b;
+    I++;
+  }
+  const int J = b + 1 + p;
+  const int k0 = b * kNB;
+  for (int e = tid; e < kNB * kNB; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    const int64_t gi = (int64_t)I * kNB + r, gj = (int64_t)J * kNB + r;
+    PI[r * kLd + c] = (gi < n) ? A[gi * n + k0 + c] : 0.0;
+    if (J != I) PJ[r * kLd + c] = (gj < n) ? A[gj * n + k0 + c] : 0.0;
+    DV[r * kLd + c] = Dinv[(int64_t)b * kNB * kNB + e];
+  }
+  __syncthreads();
+  const int w = tid >> 6, lane = tid & 63;
+  const int qr = (w >> 1) * 32, qc = (w & 1) * 32;
+  double4_t qi[2][2], qj[2][2];
+  zero_acc(qi);
+  zero_acc(qj);
+  mm_nt_tile(PI, DV, qr, qc, lane, qi);  // Q_I = P_I L_bb^{-T}
+  if (J != I) mm_nt_tile(PJ, DV, qr, qc, lane, qj);
+  __syncthreads();
+  store_acc(PI, qi, qr, qc, lane);
+  if (J != I) store_acc(PJ, qj, qr, qc, lane);
+  __syncthreads();
+  if (I == J) {
+    for (int e = tid; e < kNB * kNB; e += 256) {
+      const int r = e >> 6, c = e & 63;
+      const int64_t gi = (int64_t)I * kNB + r;
+      if (gi < n) Lf[gi * n + k0 + c] = PI[r * kLd + c];
+    }
+  }
+  double4_t cacc[2][2];
+  zero_acc(cacc);
+  mm_nt_tile(PI, (J != I) ? PJ : PI, qr, qc, lane, cacc);  // Q_I Q_J^T
+  const bool next_diag = (I == b + 1);  // then J == I as well
+  const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int r = qr + 16 * x + lk + 4 * v, c = qc + 16 * y + li;
+        const int64_t gi = (int64_t)I * kNB + r, gj = (int64_t)J * kNB + c;
+        const bool in = gi < n && gj < n;
+        const double a = in ? A[gi * n + gj] - cacc[x][y][v] : (r == c ? 1.0 : 0.0);
+        if (next_diag) DV[r * kLd + c] = a;
+        else if (in) A[gi * n + gj] = a;
+      }
+  if (!next_diag) return;
+  __syncthreads();
+  factor_tile(DV, PJ, Ys, dsq, info);
+  store_diag(DV, PJ, Lf, Dinv, n, b + 1);
+}
+
+// ---------------------------------------------------------------------------
+// W = L^{-1}, WT = W^T.  Workgroup s owns columns [16s, 16s+16), starting at
+// block row J = 16s/64 (W_J = columns of L_JJ^{-1}); below it
+//   W_I = -L_II^{-1} sum_{K=max(J, I-kband)}^{I-1} L_IK W_K,
+// with the last kband+1 block rows of the strip kept in an LDS ring.
+__global__ __launch_bounds__(256) void k_linv_strip(const double *__restrict__ Lf,
+                                                    const double *__restrict__ Dinv, int n,
+                                                    int kband, int nring,
+                                                    double *__restrict__ W,
+                                                    double *__restrict__ WT) {
+  extern __shared__ double lds[];
+  double *ring = lds;                          // [nring][64][16]
+  double *Bx = lds + (size_t)nring * kNB * kStrip;  // [64][17]
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  const int nblk = (n + kNB - 1) / kNB;
+  const int c0 = blockIdx.x * kStrip;
+  const int J = c0 / kNB, cofs = c0 - J * kNB;
+  const int gc = c0 + li;
+  {
+    double *slot = ring + (size_t)(J % nring) * kNB * kStrip;
+    for (int e = tid; e < kNB * kStrip; e += 256) {
+      const int r = e >> 4, j = e & 15;
+      const double v = Dinv[(int64_t)J * kNB * kNB + r * kNB + cofs + j];
+      slot[e] = v;
+      const int64_t gr = (int64_t)J * kNB + r, gcol = c0 + j;
+      if (gr < n && gcol < n) {
+        W[gr * n + gcol] = v;
+        WT[gcol * n + gr] = v;
+      }
+    }
+  }
+  __syncthreads();
+  for (int I = J + 1; I < nblk; I++) {
+    const int Kmin = max(J, I - kband);
+    double4_t acc = {0.0, 0.0, 0.0, 0.0};
+    const int64_t row = (int64_t)I * kNB + 16 * w + li;
+    const double *lrow = Lf + (row < n ? row : 0) * (int64_t)n;
+    for (int K = Kmin; K < I; K++) {
+      const double *slot = ring + (size_t)(K % nring) * kNB * kStrip;
+#pragma unroll 4
+      for (int k = 0; k < kNB; k += 4) {
+        const double a = row < n ? lrow[K * kNB + k + lk] : 0.0;
+        acc = mfma4(a, slot[(k + lk) * kStrip + li], acc);
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) Bx[(16 * w + lk + 4 * v) * 17 + li] = acc[v];
+    __syncthreads();
+    double4_t o = {0.0, 0.0, 0.0, 0.0};
+    const double *D = Dinv + (int64_t)I * kNB * kNB;
+    for (int k = 0; k < 16 * (w + 1); k += 4)
+      o = mfma4(D[(16 * w + li) * kNB + k + lk], Bx[(k + lk) * 17 + li], o);
+    double *slot = ring + (size_t)(I % nring) * kNB * kStrip;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int r = 16 * w + lk + 4 * v;
+      const double val = -o[v];
+      slot[r * kStrip + li] = val;
+      const int64_t gr = (int64_t)I * kNB + r;
+      if (gr < n && gc < n) {
+        W[gr * n + gc] = val;
+        WT[(int64_t)gc * n + gr] = val;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// C = WT W restricted to m >= max(i, j) (the explicit inverse A^{-1}),
+// 64x64 tiles, k staged through LDS in chunks of 16.
+__global__ __launch_bounds__(256) void k_gemm_wtw(const double *__restrict__ WT,
+                                                  const double *__restrict__ W, int n,
+                                                  double *__restrict__ C) {
+  __shared__ double As[kNB][17];
+  __shared__ double Bs[16][kNB + 1];
+  const int nblk = (n + kNB - 1) / kNB;
+  const int bi = blockIdx.x / nblk, bj = blockIdx.x % nblk;
+  const int i0 = bi * kNB, j0 = bj * kNB;
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  const int qr = (w >> 1) * 32, qc = (w & 1) * 32;
+  double4_t acc[2][2];
+  zero_acc(acc);
+  for (int m0 = std::max(i0, j0); m0 < n; m0 += 16) {
+    for (int e = tid; e < kNB * 16; e += 256) {
+      const int r = e >> 4, k = e & 15;
+      As[r][k] = (i0 + r < n && m0 + k < n) ? WT[(int64_t)(i0 + r) * n + m0 + k] : 0.0;
+      const int kk = e >> 6, c = e & 63;
+      Bs[kk][c] = (m0 + kk < n && j0 + c < n) ? W[(int64_t)(m0 + kk) * n + j0 + c] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; k += 4) {
+      double av[2], bv[2];
+#pragma unroll
+      for (int x = 0; x < 2; x++) av[x] = As[qr + 16 * x + li][k + lk];
+#pragma unroll
+      for (int y = 0; y < 2; y++) bv[y] = Bs[k + lk][qc + 16 * y + li];
+#pragma unroll
+      for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++) acc[x][y] = mfma4(av[x], bv[y], acc[x][y]);
+    }
+    __syncthreads();
   }
 #pragma unroll
   for (int x = 0; x < 2; x++)
@@ -128,135 +357,82 @@ __global__ __launch_bounds__(256) void k_band_update(double *__restrict__ A, int
     for (int y = 0; y < 2; y++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int r = i0 + qr + 16 * x + lk + 4 * v;
-        const int c = j0 + qc + 16 * y + li;
-        if (r < n && c < n && c <= r) A[(int64_t)r * n + c] -= acc[x][y][v];
+        const int r = i0 + qr + 16 * x + lk + 4 * v, c = j0 + qc + 16 * y + li;
+        if (r < n && c < n) C[(int64_t)r * n + c] = acc[x][y][v];
       }
 }
 
-// ---- 2. inverse of each 64x64 diagonal block of L (lower) ------------------
-__global__ __launch_bounds__(64) void k_trtri_diag(const double *__restrict__ A, int64_t n,
-                                                   double *__restrict__ Dinv) {
-  __shared__ double L[kNB][kNB + 1];
-  const int b = blockIdx.x, t = threadIdx.x;
-  const int k0 = b * kNB, kb = (int)std::min<int64_t>(kNB, n - k0);
-  for (int r = 0; r < kNB; r++)
-    L[r][t] = (r < kb && t < kb && t <= r) ? A[(int64_t)(k0 + r) * n + k0 + t] : (r == t ? 1.0 : 0.0);
-  __syncthreads();
-  // column t of L^{-1}: forward substitution with e_t
-  double x[kNB];
-#pragma unroll
-  for (int r = 0; r < kNB; r++) {
-    double v = (r == t) ? 1.0 : 0.0;
-#pragma unroll
-    for (int m = 0; m < r; m++) v -= L[r][m] * x[m];
-    x[r] = v / L[r][r];
-  }
-  double *D = Dinv + (int64_t)b * kNB * kNB;
-#pragma unroll
-  for (int r = 0; r < kNB; r++) D[r * kNB + t] = x[r];
+// ---------------------------------------------------------------------------
+// host side
+static size_t strip_lds_bytes(int nring) {
+  return sizeof(double) * ((size_t)nring * kNB * kStrip + (size_t)kNB * 17);
 }
 
-// ---- 3. banded solves with n right-hand sides, in place on W ---------------
-// forward: W <- L^{-1} W; backward: W <- L^{-T} W.  One workgroup per
-// 16-column strip; per 64-row block: B -= L_band B_band (GEMM over the
-// band), then B <- Dinv B.
-constexpr int kStrip = 16;
+static void *zalloc(size_t bytes) { return dalloc(bytes); }
 
-template <bool FWD>
-__global__ __launch_bounds__(256) void k_band_solve(const double *__restrict__ A, int64_t n,
-                                                    int bw, const double *__restrict__ Dinv,
-                                                    double *__restrict__ W) {
-  __shared__ double B[kNB][kStrip + 1];
-  __shared__ double Di[kNB][kNB + 1];
-  const int c0 = blockIdx.x * kStrip;
-  const int tid = threadIdx.x;
-  const int col = tid % kStrip, rg = tid / kStrip;  // 16 row groups x 4 rows
-  const int nblk = (int)((n + kNB - 1) / kNB);
-  for (int s = 0; s < nblk; s++) {
-    const int b = FWD ? s : nblk - 1 - s;
-    const int r0 = b * kNB;
-    const int kb = (int)std::min<int64_t>(kNB, n - r0);
-    // B = W[block rows, strip] - (band coupling) * W[band rows, strip]
-    for (int q = 0; q < 4; q++) {
-      const int r = rg * 4 + q;
-      const int gr = r0 + r;
-      double v = 0.0;
-      const int gc = c0 + col;
-      if (r < kb && gc < n) {
-        v = W[(int64_t)gr * n + gc];
-        if (FWD) {  // rows m in [max(0, gr-bw), r0): L[gr][m] W[m]
-          const int m0 = (int)std::max<int64_t>(0, (int64_t)gr - bw);
-          for (int m = m0; m < r0; m++) v -= A[(int64_t)gr * n + m] * W[(int64_t)m * n + gc];
-        } else {    // rows m in [r0+kb, min(n, gr+bw+1)): L[m][gr] W[m]
-          const int64_t m1 = std::min<int64_t>(n, (int64_t)gr + bw + 1);
-          for (int64_t m = r0 + kb; m < m1; m++) v -= A[m * n + gr] * W[m * n + gc];
-        }
-      }
-      B[r][col] = v;
-    }
-    const double *D = Dinv + (int64_t)b * kNB * kNB;
-    for (int e = tid; e < kNB * kNB; e += 256) {
-      const int r = e / kNB, c = e % kNB;
-      Di[r][c] = FWD ? D[r * kNB + c] : D[c * kNB + r];  // L_bb^{-1} or its transpose
-    }
-    __syncthreads();
-    for (int q = 0; q < 4; q++) {
-      const int r = rg * 4 + q;
-      const int gc = c0 + col;
-      if (r < kb && gc < n) {
-        double v = 0.0;
-        for (int m = 0; m < kb; m++) v += Di[r][m] * B[m][col];
-        W[(int64_t)(r0 + r) * n + gc] = v;
-      }
-    }
-    __syncthreads();
+bool CoarseFactor::allocate(int n_, int bw_, bool f32) {
+  n = n_;
+  bw = std::min(bw_, n_ - 1);
+  if (bw < 0) bw = 0;
+  nblk = (n + kNB - 1) / kNB;
+  kband = (bw + kNB - 1) / kNB;
+  nring = std::min(kband + 1, nblk);
+  if (strip_lds_bytes(nring) > 150 * 1024) {
+    set_error(TOP3D_ERR_ARG, "coarse solver: half bandwidth %d too wide (n=%d)", bw, n);
+    return false;
   }
+  const size_t nn = (size_t)n * n;
+  A = (double *)zalloc(sizeof(double) * nn);
+  L = (double *)zalloc(sizeof(double) * nn);
+  W = (double *)zalloc(sizeof(double) * nn);
+  WT = (double *)zalloc(sizeof(double) * nn);
+  Dinv = (double *)zalloc(sizeof(double) * (size_t)nblk * kNB * kNB);
+  info = (int *)zalloc(sizeof(int) * 4);
+  cw64 = (double *)zalloc(sizeof(double) * 2 * (size_t)n);
+  if (f32) {
+    W32 = (float *)zalloc(sizeof(float) * nn);
+    WT32 = (float *)zalloc(sizeof(float) * nn);
+    cw32 = (float *)zalloc(sizeof(float) * 2 * (size_t)n);
+  }
+  return A && L && W && WT && Dinv && info && cw64 && (!f32 || (W32 && WT32 && cw32));
 }
 
-__global__ void k_identity(double *__restrict__ W, int64_t n) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < n * n) W[e] = (e / n == e % n) ? 1.0 : 0.0;
+void CoarseFactor::release() {
+  for (void *p : {(void *)A, (void *)L, (void *)W, (void *)WT, (void *)Dinv, (void *)info,
+                  (void *)cw64, (void *)W32, (void *)WT32, (void *)cw32})
+    if (p) dfree(p);
+  A = L = W = WT = Dinv = cw64 = nullptr;
+  W32 = WT32 = cw32 = nullptr;
+  info = nullptr;
 }
 
-// A (n x n, row-major, symmetric) <- A^{-1}, via work (>= n*n + 65*64*64 +
-// 1 doubles).  Returns TOP3D_OK or TOP3D_ERR_NOT_SPD.
-int dense_cholesky_inverse_banded(double *A, int n, int bw, double *work) {
-  const int64_t nn = (int64_t)n * n;
-  double *W = work;
-  const int nblk = (n + kNB - 1) / kNB;
-  double *Dinv = work + nn;
-  int *info = (int *)(Dinv + (int64_t)nblk * kNB * kNB);
-  hipMemsetAsync(info, 0, sizeof(int), stream());
-  const int bwt = (bw + kNB - 1) / kNB + 1;  // tile half-bandwidth
-  for (int b = 0; b < nblk; b++) {
-    const int k0 = b * kNB, kb = std::min(kNB, n - k0);
-    hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(64), 0, stream(), A, (int64_t)n, k0, kb,
-                       info);
-    const int r0 = k0 + kb, r1 = std::min(n, k0 + kb + bw + 1);
-    if (r1 > r0) {
-      hipLaunchKernelGGL(k_trsm_panel, dim3((r1 - r0 + 63) / 64), dim3(64), 0, stream(), A,
-                         (int64_t)n, k0, kb, r0, r1);
-      const int t0 = b + 1, t1 = (r1 - 1) / kNB;  // trailing tiles touched
-      const int ntiles = t1 - t0 + 1;
-      if (ntiles > 0)
-        hipLaunchKernelGGL(k_band_update, dim3(ntiles * (bwt + 1)), dim3(256), 0, stream(),
-                           A, (int64_t)n, k0, kb, t0, ntiles, bwt);
-    }
+int CoarseFactor::factor() {
+  hipStream_t s = stream();
+  hipMemsetAsync(info, 0, sizeof(int), s);
+  hipLaunchKernelGGL(k_chol_step, dim3(1), dim3(256), 0, s, A, L, Dinv, n, -1, info);
+  for (int b = 0; b + 1 < nblk; b++) {
+    const int k0 = b * kNB;
+    const int t1 = std::min(nblk - 1, (k0 + kNB - 1 + bw) / kNB);
+    const int ntile = t1 - b;
+    hipLaunchKernelGGL(k_chol_step, dim3(ntile * (ntile + 1) / 2), dim3(256), 0, s, A, L, Dinv,
+                       n, b, info);
   }
-  hipLaunchKernelGGL(k_trtri_diag, dim3(nblk), dim3(64), 0, stream(), A, (int64_t)n, Dinv);
-  const int bs = 256;
-  hipLaunchKernelGGL(k_identity, dim3((nn + bs - 1) / bs), dim3(bs), 0, stream(), W,
-                     (int64_t)n);
-  const int nstrip = (n + kStrip - 1) / kStrip;
-  hipLaunchKernelGGL((k_band_solve<true>), dim3(nstrip), dim3(256), 0, stream(), A,
-                     (int64_t)n, bw, Dinv, W);
-  hipLaunchKernelGGL((k_band_solve<false>), dim3(nstrip), dim3(256), 0, stream(), A,
-                     (int64_t)n, bw, Dinv, W);
+  const size_t lds = strip_lds_bytes(nring);
+  static bool attr = false;
+  if (!attr) {
+    hipFuncSetAttribute((const void *)k_linv_strip, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        150 * 1024);
+    attr = true;
+  }
+  hipLaunchKernelGGL(k_linv_strip, dim3((n + kStrip - 1) / kStrip), dim3(256), lds, s, L, Dinv,
+                     n, kband, nring, W, WT);
+  if (W32) {
+    launch_convert<float, double>((int64_t)n * n, W, W32);
+    launch_convert<float, double>((int64_t)n * n, WT, WT32);
+  }
   int hinfo = 0;
-  hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, stream());
-  hipMemcpyAsync(A, W, sizeof(double) * nn, hipMemcpyDeviceToDevice, stream());
-  if (!T3D_HIP(hipStreamSynchronize(stream()))) return TOP3D_ERR_HIP;
+  hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (!T3D_HIP(hipStreamSynchronize(s))) return TOP3D_ERR_HIP;
   if (hinfo) {
     set_error(TOP3D_ERR_NOT_SPD, "coarse Cholesky: non-positive pivot");
     return TOP3D_ERR_NOT_SPD;
@@ -264,23 +440,97 @@ int dense_cholesky_inverse_banded(double *A, int n, int bw, double *work) {
   return TOP3D_OK;
 }
 
-int dense_cholesky_inverse(double *A, int n, double *work) {
-  return dense_cholesky_inverse_banded(A, n, n, work);
+// ---------------------------------------------------------------------------
+// Coarsest solve z = W^T (W r): gather the interior DOFs, lower-triangular
+// GEMV, upper-triangular GEMV scattered back to the halo layout (the halo of
+// z stays zero).  One wavefront per row, FP64 accumulation.
+template <typename T>
+__global__ void k_gather_interior(int nx, int ny, int nz, int wy, int wz,
+                                  const T *__restrict__ in, T *__restrict__ out) {
+  const int64_t nint = (int64_t)nx * ny * nz;
+  const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= 3 * nint) return;
+  const int64_t p = d / 3;
+  const int j = (int)(p % ny), k = (int)((p / ny) % nz), i = (int)(p / ((int64_t)ny * nz));
+  out[d] = in[3 * (((int64_t)(i + 1) * wz + (k + 1)) * wy + (j + 1)) + d % 3];
 }
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_trmv_lower(const T *__restrict__ Wm, int n,
+                                                    const T *__restrict__ x, T *__restrict__ y) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  const T *a = Wm + (int64_t)row * n;
+  double s = 0.0;
+  for (int c = lane; c <= row; c += 64) s += (double)a[c] * (double)x[c];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) y[row] = (T)s;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_trmv_upper_scatter(int nx, int ny, int nz, int wy, int wz,
+                                                            const T *__restrict__ WTm, int n,
+                                                            const T *__restrict__ y,
+                                                            T *__restrict__ out) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  const T *a = WTm + (int64_t)row * n;
+  double s = 0.0;
+  for (int c = row + lane; c < n; c += 64) s += (double)a[c] * (double)y[c];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) {
+    const int64_t p = row / 3;
+    const int j = (int)(p % ny), k = (int)((p / ny) % nz), i = (int)(p / ((int64_t)ny * nz));
+    out[3 * (((int64_t)(i + 1) * wz + (k + 1)) * wy + (j + 1)) + row % 3] = (T)s;
+  }
+}
+
+template <typename T>
+void launch_coarse_apply(const DevGrid &g, int l, const CoarseFactor &cf, const T *in, T *out) {
+  const Level &L = g.lv[l];
+  const int n = cf.n;
+  const T *Wm = cf.w<T>(), *WTm = cf.wt<T>();
+  T *x = cf.cw<T>(), *y = x + n;
+  const int bs = 256;
+  hipLaunchKernelGGL((k_gather_interior<T>), dim3((n + bs - 1) / bs), dim3(bs), 0, stream(),
+                     L.nx, L.ny, L.nz, L.wy, L.wz, in, x);
+  hipLaunchKernelGGL((k_trmv_lower<T>), dim3((n + 3) / 4), dim3(256), 0, stream(), Wm, n, x, y);
+  hipLaunchKernelGGL((k_trmv_upper_scatter<T>), dim3((n + 3) / 4), dim3(256), 0, stream(), L.nx,
+                     L.ny, L.nz, L.wy, L.wz, WTm, n, y, out);
+}
+
+template void launch_coarse_apply<double>(const DevGrid &, int, const CoarseFactor &,
+                                          const double *, double *);
+template void launch_coarse_apply<float>(const DevGrid &, int, const CoarseFactor &,
+                                         const float *, float *);
 
 }  // namespace t3d
 
+// A (n x n FP64, device, symmetric positive definite with half bandwidth bw)
+// <- A^{-1}.
 extern "C" int top3d_dense_spd_inverse(double *A_dev, int n, int bw) {
   using namespace t3d;
-  if (n <= 0 || bw < 0) {
+  if (n <= 0 || bw < 0 || !A_dev) {
     set_error(TOP3D_ERR_ARG, "top3d_dense_spd_inverse: n=%d bw=%d", n, bw);
     return TOP3D_ERR_ARG;
   }
-  const int nblk = (n + kNB - 1) / kNB;
-  double *work = (double *)dalloc(sizeof(double) * ((size_t)n * n + (size_t)nblk * kNB * kNB + 8));
-  if (!work) return TOP3D_ERR_HIP;
-  const int rc = dense_cholesky_inverse_banded(A_dev, n, std::min(bw, n), work);
-  hipStreamSynchronize(stream());
-  dfree(work);
+  CoarseFactor cf;
+  if (!cf.allocate(n, bw, false)) {
+    cf.release();
+    return top3d_last_error() ? top3d_last_error() : TOP3D_ERR_HIP;
+  }
+  hipMemcpyAsync(cf.A, A_dev, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToDevice, stream());
+  int rc = cf.factor();
+  if (rc == TOP3D_OK) {
+    const int nblk = (n + kNB - 1) / kNB;
+    hipLaunchKernelGGL(k_gemm_wtw, dim3(nblk * nblk), dim3(256), 0, stream(), cf.WT, cf.W, n,
+                       A_dev);
+    if (!T3D_HIP(hipStreamSynchronize(stream()))) rc = TOP3D_ERR_HIP;
+  }
+  cf.release();
   return rc;
 }

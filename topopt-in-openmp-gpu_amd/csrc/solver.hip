@@ -4,7 +4,7 @@
 //
 // Levels: 0 fine matrix-free, 1 matrix-free with the 8 sub-cell matrices,
 // 2..nl-2 assembled 27-point stencils (symmetric BC elimination), nl-1 the
-// explicit inverse of the FP64 Cholesky-factored coarsest matrix.  Nothing
+// FP64 Cholesky-factored coarsest matrix, applied as W^T W with W = L^{-1}.  Nothing
 // leaves the device inside the CG loop except the residual norm read each
 // iteration for the stopping test.
 #include <chrono>
@@ -72,19 +72,13 @@ bool SolverState::allocate(DevGrid *grid, int nlev) {
   const Level &C = g->lv[nl - 1];
   ncoarse = (int)(3 * C.interior_nodes());
   bandwidth = coarse_bandwidth(C);
-  const int64_t nn = (int64_t)ncoarse * ncoarse;
-  Ainv64 = alloc_t<double>(nn);
-  Ainv32 = alloc_t<float>(nn);
-  const int nblk = (ncoarse + 63) / 64;
-  dense_work = alloc_t<double>(nn + (int64_t)nblk * 64 * 64 + 8);
-  cwork64 = alloc_t<double>(ncoarse);
-  cwork32 = alloc_t<float>(ncoarse);
+  if (!coarse.allocate(ncoarse, bandwidth, true)) return false;
   scal = alloc_t<double>(16);
   partials = alloc_t<double>(1024 + 16);
   if (!T3D_HIP(hipHostMalloc((void **)&hscal, sizeof(double) * 16, hipHostMallocDefault)))
     return false;
   if (!T3D_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming))) return false;
-  return T3D_HIP(hipStreamSynchronize(stream())) && r && p && q && z && Ainv64 && dense_work;
+  return T3D_HIP(hipStreamSynchronize(stream())) && r && p && q && z;
 }
 
 void SolverState::release() {
@@ -104,7 +98,8 @@ void SolverState::release() {
     f(S32[l]);
   }
   f(r); f(p); f(q); f(z); f(E64); f(E32);
-  f(Ainv64); f(Ainv32); f(dense_work); f(cwork64); f(cwork32); f(scal); f(partials);
+  coarse.release();
+  f(scal); f(partials);
   if (hscal) hipHostFree(hscal);
   if (ev) hipEventDestroy(ev);
   magic = 0;
@@ -125,14 +120,12 @@ bool SolverState::setup(const double *x) {
     launch_node_stencil(*g, l, Ke[l], S64[l], 1);
     launch_invdiag_stencil<double>(*g, l, S64[l], invD64[l]);
   }
-  launch_stencil_to_dense(*g, nl - 1, S64[nl - 1], Ainv64, ncoarse);
-  const int rc = dense_cholesky_inverse_banded(Ainv64, ncoarse, bandwidth, dense_work);
-  if (rc != TOP3D_OK) return false;
+  launch_stencil_to_dense(*g, nl - 1, S64[nl - 1], coarse.A, ncoarse);
+  if (coarse.factor() != TOP3D_OK) return false;
   if (mixed) {
     for (int l = 0; l < nl; l++) launch_convert<float, double>(g->lv[l].ndof, invD64[l], invD32[l]);
     for (int l = 2; l < nl - 1; l++)
       launch_convert<float, double>(243 * g->lv[l].interior_nodes(), S64[l], S32[l]);
-    launch_convert<float, double>((int64_t)ncoarse * ncoarse, Ainv64, Ainv32);
   }
   return true;
 }
@@ -185,8 +178,7 @@ void SolverState::vcycle(int l) {
   op<T>(l, 2, zc, d_, r_);  // d = r - K z
   launch_restrict<T>(*g, l, d_, vec<T>(l + 1, kR));
   if (nl == l + 2) {
-    launch_symv_coarse<T>(*g, l + 1, ainv<T>(), ncoarse, vec<T>(l + 1, kR),
-                          vec<T>(l + 1, kZ), cwork<T>());
+    launch_coarse_apply<T>(*g, l + 1, coarse, vec<T>(l + 1, kR), vec<T>(l + 1, kZ));
   } else {
     vcycle<T>(l + 1);
   }

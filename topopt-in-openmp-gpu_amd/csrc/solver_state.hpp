@@ -5,8 +5,6 @@
 
 namespace t3d {
 
-int dense_cholesky_inverse_banded(double *A, int n, int bw, double *work);
-
 struct SolverState {
   static constexpr uint32_t kMagic = 0x74336453u;
   enum { kR = 0, kZ = 1, kD = 2 };
@@ -33,13 +31,9 @@ struct SolverState {
   double *Ke[kMaxLevels] = {};
   double *S64[kMaxLevels] = {};
   float *S32[kMaxLevels] = {};
-  // coarsest level: explicit inverse
+  // coarsest level: Cholesky factor and W = L^{-1}
   int ncoarse = 0, bandwidth = 0;
-  double *Ainv64 = nullptr;
-  float *Ainv32 = nullptr;
-  double *dense_work = nullptr;
-  double *cwork64 = nullptr;
-  float *cwork32 = nullptr;
+  CoarseFactor coarse;
   // device scalars, reduction partials, pinned host copy
   double *scal = nullptr, *partials = nullptr, *hscal = nullptr;
   hipEvent_t ev = nullptr;
@@ -71,12 +65,6 @@ struct SolverState {
   }
   template <typename T> const T *stencil_t(int l) {
     if constexpr (sizeof(T) == 8) return (const T *)S64[l]; else return (const T *)S32[l];
-  }
-  template <typename T> const T *ainv() {
-    if constexpr (sizeof(T) == 8) return (const T *)Ainv64; else return (const T *)Ainv32;
-  }
-  template <typename T> T *cwork() {
-    if constexpr (sizeof(T) == 8) return (T *)cwork64; else return (T *)cwork32;
   }
 };
 

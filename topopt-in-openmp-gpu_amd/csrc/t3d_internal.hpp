@@ -129,11 +129,34 @@ void launch_node_stencil(const DevGrid &g, int l, const double *Ke, double *S,
 void launch_stencil_to_dense(const DevGrid &g, int l, const double *S,
                              double *A, int n);
 
-// dense FP64 coarse solver
-int dense_cholesky_inverse(double *A, int n, double *work);  // A <- A^{-1}
+// dense FP64 coarse solver: A = L L^T (banded), W = L^{-1}, WT = W^T, so
+// that A^{-1} = WT W; FP32 copies of W and WT for the mixed V-cycle
+struct CoarseFactor {
+  int n = 0, bw = 0, nblk = 0, kband = 0, nring = 0;
+  double *A = nullptr;     // n x n matrix (lower triangle read; overwritten)
+  double *L = nullptr;     // n x n Cholesky factor
+  double *Dinv = nullptr;  // nblk 64x64 inverses of L's diagonal blocks
+  double *W = nullptr, *WT = nullptr;
+  float *W32 = nullptr, *WT32 = nullptr;
+  int *info = nullptr;
+  double *cw64 = nullptr;  // 2n work (gathered input, intermediate)
+  float *cw32 = nullptr;
+  bool allocate(int n, int bw, bool f32);
+  void release();
+  int factor();  // TOP3D_OK or TOP3D_ERR_NOT_SPD / TOP3D_ERR_HIP
+  template <typename T> const T *w() const {
+    if constexpr (sizeof(T) == 8) return (const T *)W; else return (const T *)W32;
+  }
+  template <typename T> const T *wt() const {
+    if constexpr (sizeof(T) == 8) return (const T *)WT; else return (const T *)WT32;
+  }
+  template <typename T> T *cw() const {
+    if constexpr (sizeof(T) == 8) return (T *)cw64; else return (T *)cw32;
+  }
+};
+// out (level-l halo layout) = A^{-1} in (interior DOFs)
 template <typename T>
-void launch_symv_coarse(const DevGrid &g, int l, const T *Ainv, int n,
-                        const T *in, T *out, T *work);
+void launch_coarse_apply(const DevGrid &g, int l, const CoarseFactor &cf, const T *in, T *out);
 
 // BLAS-1 (FP64 accumulation in the reductions)
 template <typename T>
