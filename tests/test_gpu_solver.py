@@ -19,6 +19,10 @@ pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
+def relerr(got, ref):
+    return np.abs(np.asarray(got) - np.asarray(ref)).max() / max(np.abs(ref).max(), 1e-300)
+
+
 def uniform(g):
     return g.density(fill=float(np.float32(0.2)))
 
@@ -150,3 +154,85 @@ def test_autograd_compliance_gradient_fd():
         fd = (float(compliance(xp_)) - float(compliance(xm_))) / (2 * h)
         assert float(grad_x[e]) == pytest.approx(fd, rel=1e-5, abs=1e-7 * abs(float(c)))
     g.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_design_loop_c2_matches_oracle_trajectory(precision):
+    """128x64x64 cantilever (coarse 16x8x8, 4 levels), 20 design iterations
+    against the oracle fixture: per-iteration compliance within 1e-8 (FP64)
+    or 1e-5 (mixed) relative, equal CG iteration counts in FP64, final
+    filtered densities within 1e-3 (fixture quantized to 1/65535)."""
+    ref = json.load(open(os.path.join(GOLDEN, "c2_fp64.json")))
+    print(f"[gpu design c2] {precision}", flush=True)
+    g = top3d_amd.Grid(ref["nelx"], ref["nely"], ref["nelz"], ref["nl"], precision=precision)
+    recs, st = top3d_amd.design_loop(g, ref["iters"])
+    assert len(recs) == len(ref["records"])
+    tol = 1e-8 if precision == "fp64" else 1e-5
+    worst = 0.0
+    for a, b in zip(recs, ref["records"]):
+        rel = abs(a["c"] - b["c"]) / abs(b["c"])
+        worst = max(worst, rel)
+        assert rel <= tol, (a, b)
+        assert abs(a["vol"] - b["vol"]) <= 1e-6
+        if precision == "fp64":
+            assert a["cgiter"] == b["cgiter"], (a, b)
+    xq = np.load(os.path.join(GOLDEN, "c2_xphys_fp64.npz"))["xphys_u16"]
+    got = st.xPhys.cpu().numpy()[g.interior_elements()]
+    dx = np.abs(got - xq.astype(np.float64) / 65535.0).max()
+    print(f"[gpu design c2] {precision}: max rel compliance diff {worst:.2e}, "
+          f"final density max diff {dx:.2e}, cg {[r['cgiter'] + 1 for r in recs]}", flush=True)
+    assert dx <= 1e-3
+    g.close()
+
+
+def test_autograd_c3_gradient_fd_and_oracle_sensitivity():
+    """Differentiation case: 64x32x32 elements, 3 levels (coarse 16x8x8),
+    densities i.i.d. uniform in [0.1, 1] (seed 2024), FP64, symmetric BCs so
+    that CG converges to 1e-12 on the heterogeneous field.  The backward's
+    element sensitivities equal the oracle's at the same displacement, and
+    the gradient through the filter matches central differences on 32
+    seeded elements to 1e-6 relative.  The differences use c = F^T u (a few
+    dozen loaded DOFs, so c is evaluated to ~1e-16 relative) with h = 1e-4:
+    2 h |g| is ~5e-8, so the ~1e-14 relative rounding of the 65k-term
+    element-energy sum, or h = 1e-6, would leave errors of a few 1e-6."""
+    nelx, nely, nelz, nl = 64, 32, 32, 3
+    print("[gpu autograd c3] 64x32x32", flush=True)
+    g = top3d_amd.Grid(nelx, nely, nelz, nl, bc="symmetric")
+    g.set_max_iter(5000)
+    o = pyoracle.Grid(nelx, nely, nelz, nl, bcsym=True)
+    rng = np.random.default_rng(2024)
+    x = g.density(rng.uniform(0.1, 1.0, nelx * nely * nelz))
+    F = g.load()
+    rmin = 1.5
+    xphys = g.filter(rmin, x)
+    xp = xphys.clone().requires_grad_(True)
+    c = top3d_amd.Compliance.apply(xp, g, F, 1e-12, 0.0)
+    c.backward()
+    # the same displacement through the plain solve; oracle sensitivities
+    U = g.zeros(0)
+    g.solve(xphys, F, U, tol=1e-12)
+    c_o, dc_o = o.sensitivity(xphys.cpu().numpy(), U.cpu().numpy())
+    idx = g.interior_elements()
+    assert abs(float(c) - c_o) <= 1e-10 * abs(c_o)
+    assert abs(float(c) - float(torch.dot(F, U))) <= 1e-10 * abs(c_o)
+    assert relerr(xp.grad.cpu().numpy()[idx], dc_o[idx]) <= 1e-10
+    grad_x = g.filter_grad(rmin, xp.grad).cpu().numpy()
+
+    def compliance(xx):
+        Ux = g.zeros(0)
+        g.solve(g.filter(rmin, xx), F, Ux, tol=1e-12)
+        return float(torch.dot(F, Ux))
+
+    pick = np.random.default_rng(7).choice(idx, 32, replace=False)
+    h = 1e-4
+    worst = 0.0
+    for e in pick:
+        xp_, xm_ = x.clone(), x.clone()
+        xp_[e] += h
+        xm_[e] -= h
+        fd = (compliance(xp_) - compliance(xm_)) / (2 * h)
+        worst = max(worst, abs(grad_x[e] - fd) / abs(fd))
+        assert grad_x[e] == pytest.approx(fd, rel=1e-6), e
+    print(f"[gpu autograd c3] worst FD rel diff {worst:.2e}", flush=True)
+    g.close()
+    o.close()

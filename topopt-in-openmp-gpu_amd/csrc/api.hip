@@ -98,13 +98,24 @@ __global__ void k_scale(int64_t n, double a, double *__restrict__ v) {
   if (i < n) v[i] *= a;
 }
 
+template <typename T>
 __global__ void k_zero_noninterior(int64_t nodes, int nx, int ny, int nz, int wy, int wz,
-                                   double *__restrict__ v) {
+                                   T *__restrict__ v) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= nodes) return;
   const int j = (int)(n % wy), k = (int)((n / wy) % wz), i = (int)(n / ((int64_t)wy * wz));
   if (i >= 1 && i <= nx && j >= 1 && j <= ny && k >= 1 && k <= nz) return;
-  v[3 * n] = v[3 * n + 1] = v[3 * n + 2] = 0.0;
+  v[3 * n] = v[3 * n + 1] = v[3 * n + 2] = T(0);
+}
+
+// zero every non-interior node of a level-l vector (the operators write the
+// interior nodes only)
+template <typename T>
+static void zero_noninterior(const DevGrid &g, int l, T *v) {
+  const Level &L = g.lv[l];
+  const int64_t nodes = L.nodes();
+  hipLaunchKernelGGL((k_zero_noninterior<T>), dim3((nodes + 255) / 256), dim3(256), 0, stream(),
+                     nodes, L.nx, L.ny, L.nz, L.wy, L.wz, v);
 }
 
 // Coarse direct solver object behind initializeCholmod/factorize/solve.
@@ -218,6 +229,7 @@ void applyStateOperatorSubspace_halo(const struct gpuNode *node, const int l, co
     launch_zero_pad_rows(*g, dout);
   } else if (l == 1) {
     launch_level1<double>(*g, kApplyMode, E, din, dout, nullptr, nullptr, 0.0, 0);
+    zero_noninterior<double>(*g, 1, dout);
   } else {
     double *S = mf_stencil(*g, E, l);
     if (!S) return;
@@ -241,6 +253,7 @@ void applyStateOperatorSubspace_halo_f32(const struct gpuNode *node, const int l
   float *E = (float *)g->work(kSlotE32, sizeof(float) * g->nelem);
   launch_young<float>(*g, dx, E);
   launch_level1<float>(*g, kApplyMode, E, din, dout, nullptr, nullptr, 0.f, 0);
+  zero_noninterior<float>(*g, 1, dout);
   sync_ok();
 }
 
@@ -472,8 +485,7 @@ void applyStateOperatorSubspaceMatrix(const struct gridContext gc, const int l,
   double *va;
   upload_csr(M, &ro, &ci, &va);
   const int bs = 256;
-  hipLaunchKernelGGL(k_zero_noninterior, dim3((L.nodes() + bs - 1) / bs), dim3(bs), 0, stream(),
-                     L.nodes(), L.nx, L.ny, L.nz, L.wy, L.wz, dout);
+  zero_noninterior<double>(*g, l, dout);
   hipLaunchKernelGGL(k_csr_apply, dim3((M.nrows + bs - 1) / bs), dim3(bs), 0, stream(), L.nx,
                      L.ny, L.nz, L.wy, L.wz, ro, ci, va, din, dout, nullptr, nullptr, 0.0,
                      (int)kApplyMode);

@@ -20,6 +20,9 @@
 // in every direction, so every owned node of a tile interior is complete
 // without atomics and the result is deterministic.
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <type_traits>
 #include <vector>
 
 #include "t3d_internal.hpp"
@@ -85,12 +88,10 @@ struct Blocks {
   T m[8][3][3];
 };
 
-// y = E * KE u for one element; u, y indexed [px][py][pz][c]
+// forward per-component 8-point Walsh-Hadamard transform:
+// h[q][c] = sum_p (-1)^{q.p} u[p][c], q = qx | qy<<1 | qz<<2
 template <typename T>
-__device__ __forceinline__ void element_apply(const Blocks<T> &B, T E, const T (&u)[2][2][2][3],
-                                              T (&y)[2][2][2][3]) {
-  // forward transform, three butterfly stages; h[q][c], q = qx | qy<<1 | qz<<2
-  T h[8][3];
+__device__ __forceinline__ void wht_fwd(const T (&u)[2][2][2][3], T (&h)[8][3]) {
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     T a[2][2][2];
@@ -117,29 +118,11 @@ __device__ __forceinline__ void element_apply(const Blocks<T> &B, T E, const T (
         h[qx | (qy << 1) | 4][c] = b2[qx][qy][0] - b2[qx][qy][1];
       }
   }
-  // block products, modulus, zero pattern of the reflection blocks
-  T g[8][3];
-#pragma unroll
-  for (int s = 0; s < 8; s++) {
-    T x[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) x[c] = h[s ^ (1 << c)][c];
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      T acc = 0;
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        const bool one_bit = s != 0 && (s & (s - 1)) == 0;
-        const bool two_bit = s != 0 && s != 7 && !one_bit;
-        bool zero = false;
-        if (one_bit) zero = (s == (1 << c)) || (s == (1 << d));
-        if (two_bit) zero = (c != d) && (!((s >> c) & 1) || !((s >> d) & 1));
-        if (!zero) acc += B.m[s][c][d] * x[d];
-      }
-      g[s ^ (1 << c)][c] = E * acc;
-    }
-  }
-  // inverse transform
+}
+
+// inverse (transpose) transform: y[p][c] = sum_q (-1)^{q.p} g[q][c]
+template <typename T>
+__device__ __forceinline__ void wht_inv(const T (&g)[8][3], T (&y)[2][2][2][3]) {
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     T a[2][2][2];  // [qx][qy][pz]
@@ -169,19 +152,166 @@ __device__ __forceinline__ void element_apply(const Blocks<T> &B, T E, const T (
   }
 }
 
+// y = E * KE u for one fine element; u, y indexed [px][py][pz][c]
+template <typename T>
+__device__ __forceinline__ void element_apply(const Blocks<T> &B, T E, const T (&u)[2][2][2][3],
+                                              T (&y)[2][2][2][3]) {
+  T h[8][3];
+  wht_fwd<T>(u, h);
+  // block products, modulus, zero pattern of the reflection blocks
+  T g[8][3];
+#pragma unroll
+  for (int s = 0; s < 8; s++) {
+    T x[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) x[c] = h[s ^ (1 << c)][c];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      T acc = 0;
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        const bool one_bit = s != 0 && (s & (s - 1)) == 0;
+        const bool two_bit = s != 0 && s != 7 && !one_bit;
+        bool zero = false;
+        if (one_bit) zero = (s == (1 << c)) || (s == (1 << d));
+        if (two_bit) zero = (c != d) && (!((s >> c) & 1) || !((s >> d) & 1));
+        if (!zero) acc += B.m[s][c][d] * x[d];
+      }
+      g[s ^ (1 << c)][c] = E * acc;
+    }
+  }
+  wht_inv<T>(g, y);
+}
+
+// ---------------------------------------------------------------------------
+// Level 1.  A level-1 element is 2x2x2 fine elements with moduli E_o
+// (o = ox | oy<<1 | oz<<2); its matrix is sum_o E_o KE1[o], KE1[o] the
+// Galerkin projection of KE onto sub-cell o.  The sub-cells are mirror images
+// of sub-cell 0, so in the transformed basis, with the index of row (q, c)
+// relabelled sigma = q ^ (1 << c),
+//   (T K T^T / 64)[(sa,ca)][(sb,cb)] = K1[(sa,ca)][(sb,cb)] * Ehat[sa ^ sb],
+//   Ehat[d] = sum_o (-1)^{d.o} E_o,
+// where K1 is the transformed sub-cell-0 matrix.  K1 has 279 non-zeros in a
+// fixed pattern (independent of the Poisson ratio; kK1Mask, checked on the
+// host), so an element product costs 279 + 192 multiply-adds plus the three
+// transforms, against 8 x 576 for the sum of sub-cell matrices.
+static constexpr uint32_t kK1Mask[24] = {
+    0x53977, 0x53977, 0x53977, 0x0,      0x653f77, 0xa7b977, 0x653f77, 0x0,
+    0xdd3977, 0x650650, 0x650650, 0xffb977, 0xa7b977, 0xdd3977, 0x0,   0xa69820,
+    0xfd3f77, 0xa69820, 0xe7bf77, 0xd92900, 0xd92900, 0xe79e70, 0xfd2f50, 0xffb920};
+constexpr int kK1Nnz = 279;
+
+__host__ __device__ constexpr int k1_offset(int row) {
+  int o = 0;
+  for (int r = 0; r < row; r++) o += __builtin_popcount(kK1Mask[r]);
+  return o;
+}
+
+template <typename T>
+struct K1Coef {
+  T v[kK1Nnz];  // non-zeros of K1, row-major over (3 sa + ca, 3 sb + cb)
+};
+
+// Host: K1 from the sub-cell matrices KE1 (8 x 576, sub-cell index
+// 4 ox + 2 oz + oy), checking the mirror relation for all 8 sub-cells and
+// the zero pattern.  Returns false if either fails.
+bool level1_hadamard(const double *KE1, double *coef) {
+  auto chi = [](int q, int n) {
+    const int d = (q & 1) * kPos[n][0] + ((q >> 1) & 1) * kPos[n][1] + ((q >> 2) & 1) * kPos[n][2];
+    return (d & 1) ? -1.0 : 1.0;
+  };
+  auto transformed = [&](const double *K, int ra, int rb) {  // ra = 3 q + c
+    const int qa = ra / 3, ca = ra % 3, qb = rb / 3, cb = rb % 3;
+    double s = 0.0;
+    for (int a = 0; a < 8; a++)
+      for (int b = 0; b < 8; b++) s += chi(qa, a) * K[24 * (3 * a + ca) + 3 * b + cb] * chi(qb, b);
+    return s / 64.0;
+  };
+  double K1[24][24], scale = 0.0;
+  for (int sa = 0; sa < 8; sa++)
+    for (int ca = 0; ca < 3; ca++)
+      for (int sb = 0; sb < 8; sb++)
+        for (int cb = 0; cb < 3; cb++) {
+          const double v = transformed(KE1, 3 * (sa ^ (1 << ca)) + ca, 3 * (sb ^ (1 << cb)) + cb);
+          K1[3 * sa + ca][3 * sb + cb] = v;
+          scale = std::fmax(scale, std::fabs(v));
+        }
+  const double tol = 1e-12 * scale;
+  int k = 0;
+  for (int r = 0; r < 24; r++)
+    for (int c = 0; c < 24; c++) {
+      if ((kK1Mask[r] >> c) & 1) coef[k++] = K1[r][c];
+      else if (std::fabs(K1[r][c]) > tol) return false;
+    }
+  if (k != kK1Nnz) return false;
+  for (int ox = 0; ox < 2; ox++)
+    for (int oy = 0; oy < 2; oy++)
+      for (int oz = 0; oz < 2; oz++) {
+        const double *K = KE1 + 576 * (4 * ox + 2 * oz + oy);
+        const int o = ox | (oy << 1) | (oz << 2);
+        for (int sa = 0; sa < 8; sa++)
+          for (int ca = 0; ca < 3; ca++)
+            for (int sb = 0; sb < 8; sb++)
+              for (int cb = 0; cb < 3; cb++) {
+                const double sgn = (__builtin_popcount(o & (sa ^ sb)) & 1) ? -1.0 : 1.0;
+                const double v = transformed(K, 3 * (sa ^ (1 << ca)) + ca, 3 * (sb ^ (1 << cb)) + cb);
+                if (std::fabs(v - sgn * K1[3 * sa + ca][3 * sb + cb]) > tol) return false;
+              }
+      }
+  return true;
+}
+
+// y = (sum_o E_o KE1[o]) u for one level-1 element, Eh = transformed moduli
+template <typename T>
+__device__ __forceinline__ void element_apply_l1(const K1Coef<T> &K, const T (&Eh)[8],
+                                                 const T (&u)[2][2][2][3], T (&y)[2][2][2][3]) {
+  T h[8][3];
+  wht_fwd<T>(u, h);
+  T H[8][3];  // relabelled: H[sigma][c] = h[sigma ^ (1 << c)][c]
+#pragma unroll
+  for (int sg = 0; sg < 8; sg++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) H[sg][c] = h[sg ^ (1 << c)][c];
+  T g[8][3];
+#pragma unroll
+  for (int sa = 0; sa < 8; sa++)
+#pragma unroll
+    for (int ca = 0; ca < 3; ca++) {
+      const int row = 3 * sa + ca;
+      T acc = 0;
+#pragma unroll
+      for (int sb = 0; sb < 8; sb++) {
+        if (((kK1Mask[row] >> (3 * sb)) & 7) == 0) continue;
+        T part = 0;
+#pragma unroll
+        for (int cb = 0; cb < 3; cb++) {
+          const int col = 3 * sb + cb;
+          if ((kK1Mask[row] >> col) & 1)
+            part += K.v[k1_offset(row) + __builtin_popcount(kK1Mask[row] & ((1u << col) - 1u))] *
+                    H[sb][cb];
+        }
+        acc += Eh[sa ^ sb] * part;
+      }
+      g[sa ^ (1 << ca)][ca] = acc;
+    }
+  wht_inv<T>(g, y);
+}
+
 struct TileArgs {
-  int nelx, nely, nelz;  // fine elements
-  int wy, wz;            // node strides
-  int few, fez;          // element strides
+  int nelx, nely, nelz;  // elements of the level
+  int wy, wz;            // node strides of the level
+  int few, fez;          // fine element strides
   int TY, TZ, NX;        // tile: element columns in y and z, node planes in x
   int nty;               // tiles along y
 };
 
-template <typename T, int MODE, bool BCSYM>
-__global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_fine_tile(
-    TileArgs a, Blocks<T> B, const T *__restrict__ E, const T *__restrict__ in,
-    T *__restrict__ out, const T *__restrict__ b, const T *__restrict__ invD, T omega,
-    const uint8_t *__restrict__ fix) {
+// LVL 0: fine level, one modulus per element; LVL 1: level 1, eight fine
+// moduli per element.  MODE: apply / Jacobi sweep / residual.
+template <typename T, int LVL, int MODE, bool BCSYM>
+__global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_tile_sweep(
+    TileArgs a, std::conditional_t<LVL == 0, Blocks<T>, K1Coef<T>> B, const T *__restrict__ E,
+    const T *__restrict__ in, T *__restrict__ out, const T *__restrict__ b,
+    const T *__restrict__ invD, T omega, const uint8_t *__restrict__ fix) {
   extern __shared__ unsigned char smem_raw[];
   T *xch = reinterpret_cast<T *>(smem_raw);  // [2][nthreads][9]
   const int nthr = a.TY * a.TZ;
@@ -200,6 +330,8 @@ __global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_fine_tile(
   const bool last_y = (ty == a.TY - 1), last_z = (tz == a.TZ - 1);
   const bool own_ok = elem_ok && (!last_y || j + 1 == a.nely + 1) &&
                       (!last_z || k + 1 == a.nelz + 1);
+  // level 1: the element is real (its fine moduli exist)
+  const bool real_yz = elem_ok && j >= 1 && k >= 1;
   const int64_t sx = (int64_t)a.wy * a.wz;
   const int64_t n00 = (int64_t)k * a.wy + j;  // node (j, k) within a plane
   const int64_t noff[2][2] = {{0, a.wy}, {1, (int64_t)a.wy + 1}};  // [py][pz]
@@ -245,10 +377,47 @@ __global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_fine_tile(
 #pragma unroll
         for (int c = 0; c < 3; c++) u[0][py][pz][c] = u[1][py][pz][c];
     load_plane(1, i + 1);
-    T Ee = 0;
-    if (elem_ok && i <= a.nelx + 1) Ee = E[((int64_t)i * a.fez + k) * a.few + j];
     T y[2][2][2][3];
-    element_apply<T>(B, Ee, u, y);
+    if constexpr (LVL == 0) {
+      T Ee = 0;
+      if (elem_ok && i <= a.nelx + 1) Ee = E[((int64_t)i * a.fez + k) * a.few + j];
+      element_apply<T>(B, Ee, u, y);
+    } else {
+      T Eo[8];
+#pragma unroll
+      for (int o = 0; o < 8; o++) Eo[o] = 0;
+      if (real_yz && i >= 1 && i <= a.nelx) {
+#pragma unroll
+        for (int o = 0; o < 8; o++) {
+          const int64_t fe = ((int64_t)(2 * i - 1 + (o & 1)) * a.fez + (2 * k - 1 + (o >> 2))) *
+                                 a.few + (2 * j - 1 + ((o >> 1) & 1));
+          Eo[o] = E[fe];
+        }
+      }
+      // Ehat = per-element 8-point transform of the moduli
+      T Eh[8];
+      {
+        T p1[8], p2[8];
+#pragma unroll
+        for (int o = 0; o < 8; o += 2) {
+          p1[o] = Eo[o] + Eo[o + 1];
+          p1[o + 1] = Eo[o] - Eo[o + 1];
+        }
+#pragma unroll
+        for (int o = 0; o < 8; o += 4)
+#pragma unroll
+          for (int q = 0; q < 2; q++) {
+            p2[o + q] = p1[o + q] + p1[o + q + 2];
+            p2[o + q + 2] = p1[o + q] - p1[o + q + 2];
+          }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          Eh[q] = p2[q] + p2[q + 4];
+          Eh[q + 4] = p2[q] - p2[q + 4];
+        }
+      }
+      element_apply_l1<T>(B, Eh, u, y);
+    }
     if (i >= i0) {
       // plane i: x- face of slab i plus the pending x+ face of slab i-1
       T tot[2][2][3];
@@ -283,29 +452,27 @@ __global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_fine_tile(
           for (int c = 0; c < 3; c++) kin[c] += buf[9 * (t + a.TY + 1) + 6 + c];
         }
         const int jn = j + 1, kn = k + 1;
-        if (jn >= 1 && kn >= 1) {  // owned node is a real node (i is real)
-          const int64_t nd = i * sx + (int64_t)kn * a.wy + jn;
-          const uint8_t m = fix[nd];
+        const int64_t nd = i * sx + (int64_t)kn * a.wy + jn;
+        const uint8_t m = fix[nd];
 #pragma unroll
-          for (int c = 0; c < 3; c++) {
-            const T uc = u[0][1][1][c];
-            T kv = kin[c];
-            T o;
-            if (BCSYM) {  // the loaded value is masked; the BC row needs in
-              if ((m >> c) & 1) kv = in[3 * nd + c];
-            } else {
-              if ((m >> c) & 1) kv = uc;
-            }
-            if (MODE == kModeApply) {
-              o = kv;
-            } else if (MODE == kModeJacobi) {
-              const T uo = BCSYM ? in[3 * nd + c] : uc;
-              o = uo + omega * invD[3 * nd + c] * (b[3 * nd + c] - kv);
-            } else {
-              o = b[3 * nd + c] - kv;
-            }
-            out[3 * nd + c] = o;
+        for (int c = 0; c < 3; c++) {
+          const T uc = u[0][1][1][c];
+          T kv = kin[c];
+          T o;
+          if (BCSYM) {  // the loaded value is masked; the BC row needs in
+            if ((m >> c) & 1) kv = in[3 * nd + c];
+          } else {
+            if ((m >> c) & 1) kv = uc;
           }
+          if (MODE == kModeApply) {
+            o = kv;
+          } else if (MODE == kModeJacobi) {
+            const T uo = BCSYM ? in[3 * nd + c] : uc;
+            o = uo + omega * invD[3 * nd + c] * (b[3 * nd + c] - kv);
+          } else {
+            o = b[3 * nd + c] - kv;
+          }
+          out[3 * nd + c] = o;
         }
       }
     }
@@ -320,63 +487,97 @@ __global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_fine_tile(
 
 // ---------------------------------------------------------------------------
 // host side
-static bool g_blocks_ok = false;
-static Blocks<double> g_b64;
-static Blocks<float> g_b32;
+struct HostCoef {
+  bool done = false, ok0 = false, ok1 = false;
+  Blocks<double> b64;
+  Blocks<float> b32;
+  K1Coef<double> k64;
+  K1Coef<float> k32;
+};
+static HostCoef g_coef;
 
-static void init_blocks(const DevGrid &g) {
-  if (g_blocks_ok) return;
-  std::vector<double> KE(576);
+static void init_coef() {
+  if (g_coef.done) return;
+  g_coef.done = true;
+  std::vector<double> KE(576), KE1(8 * 576);
   element_matrices(0.3, 0, KE.data());
+  element_matrices(0.3, 1, KE1.data());
   double M[8][3][3];
-  if (!hadamard_blocks(KE.data(), M)) {
-    set_error(TOP3D_ERR_ARG, "element matrix lacks the reflection structure");
-    return;
-  }
-  for (int s = 0; s < 8; s++)
-    for (int c = 0; c < 3; c++)
-      for (int d = 0; d < 3; d++) {
-        g_b64.m[s][c][d] = M[s][c][d];
-        g_b32.m[s][c][d] = (float)M[s][c][d];
-      }
-  g_blocks_ok = true;
-  (void)g;
+  g_coef.ok0 = hadamard_blocks(KE.data(), M);
+  if (g_coef.ok0)
+    for (int s = 0; s < 8; s++)
+      for (int c = 0; c < 3; c++)
+        for (int d = 0; d < 3; d++) {
+          g_coef.b64.m[s][c][d] = M[s][c][d];
+          g_coef.b32.m[s][c][d] = (float)M[s][c][d];
+        }
+  double coef[kK1Nnz];
+  g_coef.ok1 = level1_hadamard(KE1.data(), coef);
+  if (g_coef.ok1)
+    for (int q = 0; q < kK1Nnz; q++) {
+      g_coef.k64.v[q] = coef[q];
+      g_coef.k32.v[q] = (float)coef[q];
+    }
+}
+
+bool tile_level1_available() {
+  init_coef();
+  return g_coef.ok1;
 }
 
 struct TileShape {
   int TY, TZ, NX, nty, ntz, ntx;
 };
 
-static TileShape tile_shape(const DevGrid &g, int elem_bytes) {
-  const Level &L = g.lv[0];
+// Tile of TY x TZ element columns marching over NX node planes.  LDS holds
+// two exchange buffers of 9 values per thread (under 60 KiB).  Tuning
+// override: TOP3D_TILE_L<level>=TY,TZ,NX.
+static TileShape tile_shape(const Level &L, int lvl, int elem_bytes) {
   TileShape s;
-  // LDS: two exchange buffers of 9 values per thread, kept under 60 KiB
   const int lds_threads = 60 * 1024 / (18 * elem_bytes);
   const int max_threads = std::min(elem_bytes == 4 ? 1024 : 512, lds_threads);
   const int ycap = elem_bytes == 4 ? 129 : 65;
   s.TY = std::max(2, std::min(L.nely + 1, ycap));
-  s.TZ = std::max(2, std::min(max_threads / s.TY, 16));
+  const int zcap = lvl == 0 ? 16 : 8;
+  s.TZ = std::max(2, std::min(max_threads / s.TY, zcap));
+  int nx_min = 4;
+  const char *env = std::getenv(lvl == 0 ? "TOP3D_TILE_L0" : "TOP3D_TILE_L1");
+  int oy = 0, oz = 0, ox = 0;
+  if (env && std::sscanf(env, "%d,%d,%d", &oy, &oz, &ox) == 3 && oy >= 2 && oz >= 2 && ox >= 1 &&
+      oy * oz <= max_threads) {
+    s.TY = std::min(oy, std::max(2, L.nely + 1));
+    s.TZ = std::min(oz, std::max(2, L.nelz + 1));
+    nx_min = ox;
+  }
   // a tile owns TY-1 node rows, plus the last row of the grid (its upper
   // neighbour element is halo)
   auto tiles = [](int nodes, int T) { return nodes <= T ? 1 : (nodes - T + T - 2) / (T - 1) + 1; };
   s.nty = tiles(L.ny, s.TY);
   s.ntz = tiles(L.nz, s.TZ);
-  // about two workgroups per CU (256 CUs); longer x-marches waste less
   const int64_t yz = (int64_t)s.nty * s.ntz;
-  int ntx = (int)std::max<int64_t>(1, (512 + yz - 1) / yz);
-  ntx = std::min(ntx, L.nx);
-  s.NX = (L.nx + ntx - 1) / ntx;
-  if (s.NX < 4) s.NX = std::min(4, L.nx);
+  if (env && ox >= 1) {
+    s.NX = std::min(ox, L.nx);
+  } else {
+    // about two workgroups per CU (256 CUs); longer x-marches waste less
+    int ntx = (int)std::max<int64_t>(1, (512 + yz - 1) / yz);
+    ntx = std::min(ntx, L.nx);
+    s.NX = (L.nx + ntx - 1) / ntx;
+    if (s.NX < nx_min) s.NX = std::min(nx_min, L.nx);
+  }
   s.ntx = (L.nx + s.NX - 1) / s.NX;
   return s;
 }
 
-template <typename T>
-void launch_fine_tile(const DevGrid &g, int mode, const T *E, const T *in, T *out,
-                      const T *b, const T *invD, T omega, int bcsym) {
-  init_blocks(g);
-  const Level &L = g.lv[0];
-  const TileShape s = tile_shape(g, (int)sizeof(T));
+template <typename T, int LVL>
+static void launch_tile(const DevGrid &g, int mode, const T *E, const T *in, T *out, const T *b,
+                        const T *invD, T omega, int bcsym) {
+  init_coef();
+  if (!(LVL == 0 ? g_coef.ok0 : g_coef.ok1)) {
+    set_error(TOP3D_ERR_ARG, "element matrix lacks the reflection structure");
+    return;
+  }
+  const Level &L = g.lv[LVL];
+  const TileShape s = tile_shape(L, LVL, (int)sizeof(T));
   TileArgs a;
   a.nelx = L.nelx;
   a.nely = L.nely;
@@ -393,11 +594,15 @@ void launch_fine_tile(const DevGrid &g, int mode, const T *E, const T *in, T *ou
   const int bs = (nthr + 63) / 64 * 64;
   const size_t shm = sizeof(T) * 2 * 9 * (size_t)nthr;
   dim3 grid(s.ntx, s.nty * s.ntz);
-  Blocks<T> B;
-  if constexpr (sizeof(T) == 8) B = g_b64; else B = g_b32;
-#define T3D_LAUNCH(M, S)                                                                   \
-  hipLaunchKernelGGL((k_fine_tile<T, M, S>), grid, dim3(bs), shm, stream(), a, B, E, in, \
-                     out, b, invD, omega, g.fixmask[0])
+  std::conditional_t<LVL == 0, Blocks<T>, K1Coef<T>> B;
+  if constexpr (LVL == 0) {
+    if constexpr (sizeof(T) == 8) B = g_coef.b64; else B = g_coef.b32;
+  } else {
+    if constexpr (sizeof(T) == 8) B = g_coef.k64; else B = g_coef.k32;
+  }
+#define T3D_LAUNCH(M, S)                                                                    \
+  hipLaunchKernelGGL((k_tile_sweep<T, LVL, M, S>), grid, dim3(bs), shm, stream(), a, B, E, \
+                     in, out, b, invD, omega, g.fixmask[LVL])
   if (bcsym) {
     if (mode == kModeApply) T3D_LAUNCH(kModeApply, true);
     else if (mode == kModeJacobi) T3D_LAUNCH(kModeJacobi, true);
@@ -408,6 +613,18 @@ void launch_fine_tile(const DevGrid &g, int mode, const T *E, const T *in, T *ou
     else T3D_LAUNCH(kModeResidual, false);
   }
 #undef T3D_LAUNCH
+}
+
+template <typename T>
+void launch_fine_tile(const DevGrid &g, int mode, const T *E, const T *in, T *out,
+                      const T *b, const T *invD, T omega, int bcsym) {
+  launch_tile<T, 0>(g, mode, E, in, out, b, invD, omega, bcsym);
+}
+
+template <typename T>
+void launch_level1_tile(const DevGrid &g, int mode, const T *E, const T *in, T *out,
+                        const T *b, const T *invD, T omega, int bcsym) {
+  launch_tile<T, 1>(g, mode, E, in, out, b, invD, omega, bcsym);
 }
 
 // rows j > nely+1 of every (i, k) set to zero, as the reference's operator
@@ -433,5 +650,9 @@ template void launch_fine_tile<double>(const DevGrid &, int, const double *, con
                                        double *, const double *, const double *, double, int);
 template void launch_fine_tile<float>(const DevGrid &, int, const float *, const float *,
                                       float *, const float *, const float *, float, int);
+template void launch_level1_tile<double>(const DevGrid &, int, const double *, const double *,
+                                         double *, const double *, const double *, double, int);
+template void launch_level1_tile<float>(const DevGrid &, int, const float *, const float *,
+                                        float *, const float *, const float *, float, int);
 
 }  // namespace t3d

@@ -257,9 +257,23 @@ __global__ __launch_bounds__(256) void k_level1(
   }
 }
 
+// node-centric level-1 variant kept for cross-checking (TOP3D_LEVEL1_KERNEL=node)
+static bool use_node_level1() {
+  static int v = -1;
+  if (v < 0) {
+    const char *e = std::getenv("TOP3D_LEVEL1_KERNEL");
+    v = (e && std::string(e) == "node") ? 1 : 0;
+  }
+  return v == 1;
+}
+
 template <typename T>
 void launch_level1(const DevGrid &g, int mode, const T *E, const T *in, T *out,
                    const T *b, const T *invD, T omega, int bcsym) {
+  if (!use_node_level1() && tile_level1_available()) {
+    launch_level1_tile<T>(g, mode, E, in, out, b, invD, omega, bcsym);
+    return;
+  }
   const Level &L = g.lv[1];
   const LevelArgs a = g.args(1);
   const T *KE = nullptr;

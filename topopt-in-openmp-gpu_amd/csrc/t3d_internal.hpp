@@ -94,6 +94,10 @@ void launch_fine(const DevGrid &g, int mode, const T *E, const T *in, T *out,
 template <typename T>
 void launch_fine_tile(const DevGrid &g, int mode, const T *E, const T *in, T *out,
                       const T *b, const T *invD, T omega, int bcsym);
+template <typename T>
+void launch_level1_tile(const DevGrid &g, int mode, const T *E, const T *in, T *out,
+                        const T *b, const T *invD, T omega, int bcsym);
+bool tile_level1_available();  // the level-1 element matrices have the mirror structure
 // zero the padded y rows (j > nely+1) of a level-0 vector
 void launch_zero_pad_rows(const DevGrid &g, double *out);
 // level-1 matrix-free operator with fine moduli, same modes
