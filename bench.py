@@ -93,6 +93,9 @@ def main():
     ap.add_argument("--compare-fp64", type=int, default=1)
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--kernel-only", default="",
+                    help="profiling aid: one mixed design iteration, then only REPS launches of "
+                         "this kernel (and a 256 MiB device copy as a byte-count reference)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; --gpus N > 1 is not supported "
@@ -105,6 +108,19 @@ def main():
     f = 1 << (args.levels - 1)
     nelx, nely, nelz, nl = cx * f, cy * f, cz * f, args.levels
     dofs = 3 * (nelx + 1) * (nely + 1) * (nelz + 1)
+
+    if args.kernel_only:
+        g, _, _, _ = run_form(top3d_amd, torch, nelx, nely, nelz, nl, "mixed", 0, 1)
+        ms, by = g.time_kernel(args.kernel_only, args.kernel_reps)
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"kernel": args.kernel_only, "avg_ms": ms, "bytes": by,
+                          "copy_bytes_each_way": src.numel() * 4}), flush=True)
+        g.close()
+        return
 
     g, st, recs, wall = run_form(top3d_amd, torch, nelx, nely, nelz, nl, "mixed",
                                  args.warmup, args.steps)

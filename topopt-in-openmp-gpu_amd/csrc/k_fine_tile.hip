@@ -536,18 +536,27 @@ static TileShape tile_shape(const Level &L, int lvl, int elem_bytes) {
   TileShape s;
   const int lds_threads = 60 * 1024 / (18 * elem_bytes);
   const int max_threads = std::min(elem_bytes == 4 ? 1024 : 512, lds_threads);
-  const int ycap = elem_bytes == 4 ? 129 : 65;
-  s.TY = std::max(2, std::min(L.nely + 1, ycap));
-  const int zcap = lvl == 0 ? 16 : 8;
-  s.TZ = std::max(2, std::min(max_threads / s.TY, zcap));
-  int nx_min = 4;
+  // Shapes measured on MI355X at 256x128x128 (fine) / 128x64x64 (level 1):
+  // small workgroups (200-300 threads) for the fine level, so that several
+  // share a CU and their barriers interleave; wider, short x-marches for
+  // level 1.  ycap/tz bound the tile, nx fixes the x-march (0: from the
+  // workgroup target).
+  struct Rule {
+    int ycap, tz, nx, wg_target;
+  };
+  static constexpr Rule kRules[2][2] = {{{33, 6, 0, 2000}, {33, 8, 0, 1000}},   // level 0: f64, f32
+                                        {{65, 6, 8, 0}, {65, 12, 4, 0}}};        // level 1
+  const Rule r = kRules[lvl == 0 ? 0 : 1][elem_bytes == 4 ? 1 : 0];
+  s.TY = std::max(2, std::min(L.nely + 1, r.ycap));
+  s.TZ = std::max(2, std::min({L.nelz + 1, r.tz, max_threads / s.TY}));
+  int nx_fixed = r.nx;
   const char *env = std::getenv(lvl == 0 ? "TOP3D_TILE_L0" : "TOP3D_TILE_L1");
   int oy = 0, oz = 0, ox = 0;
   if (env && std::sscanf(env, "%d,%d,%d", &oy, &oz, &ox) == 3 && oy >= 2 && oz >= 2 && ox >= 1 &&
       oy * oz <= max_threads) {
     s.TY = std::min(oy, std::max(2, L.nely + 1));
     s.TZ = std::min(oz, std::max(2, L.nelz + 1));
-    nx_min = ox;
+    nx_fixed = ox;
   }
   // a tile owns TY-1 node rows, plus the last row of the grid (its upper
   // neighbour element is halo)
@@ -555,14 +564,13 @@ static TileShape tile_shape(const Level &L, int lvl, int elem_bytes) {
   s.nty = tiles(L.ny, s.TY);
   s.ntz = tiles(L.nz, s.TZ);
   const int64_t yz = (int64_t)s.nty * s.ntz;
-  if (env && ox >= 1) {
-    s.NX = std::min(ox, L.nx);
+  if (nx_fixed > 0) {
+    s.NX = std::min(nx_fixed, L.nx);
   } else {
-    // about two workgroups per CU (256 CUs); longer x-marches waste less
-    int ntx = (int)std::max<int64_t>(1, (512 + yz - 1) / yz);
+    int ntx = (int)std::max<int64_t>(1, (r.wg_target + yz - 1) / yz);
     ntx = std::min(ntx, L.nx);
-    s.NX = (L.nx + ntx - 1) / ntx;
-    if (s.NX < nx_min) s.NX = std::min(nx_min, L.nx);
+    s.NX = std::max(4, (L.nx + ntx - 1) / ntx);
+    s.NX = std::min(s.NX, L.nx);
   }
   s.ntx = (L.nx + s.NX - 1) / s.NX;
   return s;
