@@ -192,6 +192,7 @@ void release_dev_grid(const struct gridContext *gc) {
     dfree(g->KE[i]);
     dfree(g->KEf[i]);
   }
+  dfree(g->KE2);
   for (int i = 0; i < 8; i++) dfree(g->scratch[i]);
   delete g;
 }
@@ -257,6 +258,13 @@ static DevGrid *build_dev_grid(struct gridContext *gc, int nl) {
     T3D_HIP(hipMemcpyAsync(g->KE[i], gc->precomputedKE[i], sizeof(double) * cnt,
                            hipMemcpyHostToDevice, stream()));
     T3D_HIP(hipMemcpyAsync(g->KEf[i], f.data(), sizeof(float) * cnt,
+                           hipMemcpyHostToDevice, stream()));
+    T3D_HIP(hipStreamSynchronize(stream()));
+  }
+  if (nl >= 3) {  // level-2 sub-cell matrices: coarse element matrices as one GEMM
+    const size_t cnt = 576 * 64;
+    g->KE2 = (double *)dalloc(sizeof(double) * cnt);
+    T3D_HIP(hipMemcpyAsync(g->KE2, gc->precomputedKE[2], sizeof(double) * cnt,
                            hipMemcpyHostToDevice, stream()));
     T3D_HIP(hipStreamSynchronize(stream()));
   }

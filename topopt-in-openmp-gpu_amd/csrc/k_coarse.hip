@@ -94,9 +94,66 @@ __global__ __launch_bounds__(576) void k_galerkin(
   Kparent[576 * pe + t] = acc;
 }
 
+// Level-2 element matrices as one GEMM on FP64 MFMA:
+//   Ke2[e][q] = sum_{f<64} E[fine(e, f)] KE_2[f][q],  q < 576,
+// with f = 16 ii + 4 kk + jj the reference's sub-cell index
+// (local_matrix.c:283).  A workgroup owns 128 level-2 elements: their 64
+// moduli are gathered into LDS, and each wave sweeps 9 of the 36 16-column
+// tiles of KE_2, holding the tile's 64x16 slice in registers while it runs
+// over the 8 16-element row tiles (v_mfma_f64_16x16x4f64, K = 64).
+constexpr int kL2Rows = 128;
+
+__global__ __launch_bounds__(256) void k_elem_level2_mfma(
+    int nelx, int nely, int nelz, int few, int fez, const double *__restrict__ KE2,
+    const double *__restrict__ E, double *__restrict__ Ke2) {
+  __shared__ double As[kL2Rows][65];
+  const int64_t nel = (int64_t)nelx * nely * nelz;
+  const int64_t e0 = (int64_t)blockIdx.x * kL2Rows;
+  const int tid = threadIdx.x;
+  for (int q = tid; q < kL2Rows * 64; q += 256) {
+    const int r = q >> 6, f = q & 63;
+    const int64_t e = e0 + r;
+    double v = 0.0;
+    if (e < nel) {
+      const int J = (int)(e % nely), K = (int)((e / nely) % nelz),
+                I = (int)(e / ((int64_t)nely * nelz));
+      const int ii = f >> 4, kk = (f >> 2) & 3, jj = f & 3;
+      v = E[((int64_t)(4 * I + ii + 1) * fez + (4 * K + kk + 1)) * few + (4 * J + jj + 1)];
+    }
+    As[r][f] = v;
+  }
+  __syncthreads();
+  const int w = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
+  typedef double double4_v __attribute__((ext_vector_type(4)));
+  for (int nt = w; nt < 36; nt += 4) {
+    double bfrag[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) bfrag[k] = KE2[(4 * k + lk) * 576 + 16 * nt + li];
+#pragma unroll 2
+    for (int mt = 0; mt < kL2Rows / 16; mt++) {
+      double4_v acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k = 0; k < 16; k++)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(As[16 * mt + li][4 * k + lk], bfrag[k], acc,
+                                                   0, 0, 0);
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int64_t e = e0 + 16 * mt + lk + 4 * v;
+        if (e < nel) Ke2[e * 576 + 16 * nt + li] = acc[v];
+      }
+    }
+  }
+}
+
 void launch_elem_level2(const DevGrid &g, const double *E, double *Ke2) {
   const Level &L = g.lv[2];
-  hipLaunchKernelGGL((k_galerkin<0>), dim3(L.elements()), dim3(576), 0, stream(), L.nelx,
+  const int64_t nel = L.elements();
+  if (g.KE2) {
+    hipLaunchKernelGGL(k_elem_level2_mfma, dim3((nel + kL2Rows - 1) / kL2Rows), dim3(256), 0,
+                       stream(), L.nelx, L.nely, L.nelz, g.ewy, g.ewz, g.KE2, E, Ke2);
+    return;
+  }
+  hipLaunchKernelGGL((k_galerkin<0>), dim3(nel), dim3(576), 0, stream(), L.nelx,
                      L.nely, L.nelz, 0, 0, g.ewy, g.ewz, g.KE[1], E, nullptr, Ke2);
 }
 

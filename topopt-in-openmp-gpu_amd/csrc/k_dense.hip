@@ -72,65 +72,124 @@ __device__ inline void store_acc(double *X, const double4_t acc[2][2], int qr, i
 }
 
 // ---------------------------------------------------------------------------
-// Cholesky and inverse of one 64x64 SPD tile in LDS, 256 threads.
+// Cholesky and inverse of one 64x64 SPD tile in LDS, 256 threads (4 waves),
+// blocked by 16: per diagonal block kb
+//   (a) wave 0, lanes 0..15 (one row each, broadcasts through LDS): the 16x16
+//       Cholesky L_D and its inverse X_D;
+//   (b) the panel below: P <- P X_D^T;
+//   (c) the trailing lower triangle: C -= P P^T as 16x16 MFMA tiles;
+// then the off-diagonal 16x16 blocks of the inverse by distance.
 // In: T (lower triangle; rows/cols beyond the matrix padded with identity).
 // Out: T = L (zeros above the diagonal), X = L^{-1}.  A non-positive pivot
-// sets *info (and is replaced by 1 so that the sweep finishes).
-__device__ void factor_tile(double *T, double *X, double *Ys, double *dsq, int *info) {
-  const int t = threadIdx.x;
-  const int r = t >> 2, s = t & 3;  // thread owns (r, 4q+s), q = 0..15
-  double v[16];
-#pragma unroll
-  for (int q = 0; q < 16; q++) {
-    const int c = 4 * q + s;
-    v[q] = (c <= r) ? T[r * kLd + c] : 0.0;
-  }
-  // right-looking with deferred column scaling: a_rc -= a_rj a_cj / a_jj;
-  // column j+1 is final after step j and is published to T by its owners
-#pragma unroll
-  for (int j = 0; j < kNB; j++) {
-    double ajj = T[j * kLd + j];
-    if (!(ajj > 0.0)) {
-      if (t == 0) info[0] = 1;
-      ajj = 1.0;
-    }
-    const double f = T[r * kLd + j] / ajj;
-#pragma unroll
-    for (int q = 0; q < 16; q++) {
-      const int c = 4 * q + s;
-      if (c > j && c <= r) v[q] -= f * T[c * kLd + j];
-    }
-    if (j + 1 < kNB && s == ((j + 1) & 3) && r >= j + 1) T[r * kLd + j + 1] = v[(j + 1) >> 2];
-    __syncthreads();
-  }
-  if (t < kNB) {
-    const double d = T[t * kLd + t];
-    dsq[t] = sqrt(d > 0.0 ? d : 1.0);
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < 16; q++) {
-    const int c = 4 * q + s;
-    T[r * kLd + c] = (c <= r) ? v[q] / dsq[c] : 0.0;
-  }
+// sets *info (and is replaced by 1 so that the factorization finishes).
+// Orders the LDS accesses of one wavefront (lanes of a wave execute their LDS
+// instructions in order; the fences keep the compiler from moving them).
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void factor_tile(double *T, double *X, double *Ys, double *wbuf,
+                                            int *info) {
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  const int li = lane & 15, lk = lane >> 4;
   for (int e = t; e < kNB * kNB; e += 256) X[(e >> 6) * kLd + (e & 63)] = 0.0;
   __syncthreads();
-  // inverse: the four 16x16 diagonal blocks by substitution (one wave each)
-  const int w = t >> 6, lane = t & 63;
-  if (lane < 16) {
-    const int b0 = 16 * w, c = lane;
-    double x[16];
+  for (int kb = 0; kb < 4; kb++) {
+    const int d0 = 16 * kb;
+    if (t < 16) {  // (a): one row per lane, broadcasts through LDS
+      const int r = t;
+      double v[16];
 #pragma unroll
-    for (int i = 0; i < 16; i++) {
-      double acc = (i == c) ? 1.0 : 0.0;
+      for (int c = 0; c < 16; c++) v[c] = (c <= r) ? T[(d0 + r) * kLd + d0 + c] : 0.0;
 #pragma unroll
-      for (int m = 0; m < i; m++) acc -= T[(b0 + i) * kLd + b0 + m] * x[m];
-      x[i] = acc / T[(b0 + i) * kLd + b0 + i];
-      X[(b0 + i) * kLd + b0 + c] = x[i];
+      for (int j = 0; j < 16; j++) {
+        if (r == j) wbuf[16] = v[j];
+        wave_lds_sync();
+        double piv = wbuf[16];
+        if (!(piv > 0.0)) {
+          if (r == 0) info[0] = 1;
+          piv = 1.0;
+        }
+        const double ljj = sqrt(piv);
+        const double lrj = (r == j) ? ljj : v[j] / ljj;
+        v[j] = (r >= j) ? lrj : 0.0;
+        wbuf[r] = lrj;
+        wave_lds_sync();
+#pragma unroll
+        for (int c = j + 1; c < 16; c++)
+          if (c <= r) v[c] -= lrj * wbuf[c];
+        wave_lds_sync();
+      }
+#pragma unroll
+      for (int c = 0; c < 16; c++) T[(d0 + r) * kLd + d0 + c] = v[c];
+      wave_lds_sync();
+      // column r of X_D = L_D^{-1}: forward substitution, rows of L_D read
+      // by all lanes from LDS
+      double x[16];
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        double acc = (i == r) ? 1.0 : 0.0;
+#pragma unroll
+        for (int m = 0; m < i; m++) acc -= T[(d0 + i) * kLd + d0 + m] * x[m];
+        x[i] = acc / T[(d0 + i) * kLd + d0 + i];
+        X[(d0 + i) * kLd + d0 + r] = x[i];
+      }
+    }
+    __syncthreads();
+    const int npan = kNB - d0 - 16;  // panel rows below the diagonal block
+    if (npan > 0) {
+      // (b) P[p][c] <- sum_{m <= c} P[p][m] X_D[c][m]
+      double pv[3];
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        const int e = t + 256 * q;
+        pv[q] = 0.0;
+        if (e < npan * 16) {
+          const int p = d0 + 16 + (e >> 4), c = e & 15;
+          double acc = 0.0;
+          for (int m = 0; m <= c; m++) acc += T[p * kLd + d0 + m] * X[(d0 + c) * kLd + d0 + m];
+          pv[q] = acc;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        const int e = t + 256 * q;
+        if (e < npan * 16) T[(d0 + 16 + (e >> 4)) * kLd + d0 + (e & 15)] = pv[q];
+      }
+      __syncthreads();
+      // (c) trailing tiles (ti, tj), kb < tj <= ti < 4, round-robin over waves
+      const int nt = 3 - kb, ntiles = nt * (nt + 1) / 2;
+      for (int q = w; q < ntiles; q += 4) {
+        int ti = 0, tj = 0, cnt = 0;
+        for (int a = 0; a < nt; a++)
+          for (int b2 = 0; b2 <= a; b2++) {
+            if (cnt == q) {
+              ti = kb + 1 + a;
+              tj = kb + 1 + b2;
+            }
+            cnt++;
+          }
+        double4_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < 16; k += 4)
+          acc = mfma4(T[(16 * ti + li) * kLd + d0 + k + lk], T[(16 * tj + li) * kLd + d0 + k + lk],
+                      acc);
+#pragma unroll
+        for (int v = 0; v < 4; v++) T[(16 * ti + lk + 4 * v) * kLd + 16 * tj + li] -= acc[v];
+      }
+      __syncthreads();
     }
   }
+  for (int e = t; e < kNB * kNB; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    if (c > r) T[r * kLd + c] = 0.0;
+  }
   __syncthreads();
-  // off-diagonal blocks by distance d: X_IJ = -X_II sum_{K=J}^{I-1} L_IK X_KJ
+  // off-diagonal blocks of the inverse by distance d:
+  //   X_IJ = -X_II sum_{K=J}^{I-1} L_IK X_KJ
   const int rr = t >> 4, cc = t & 15;
   for (int d = 1; d < 4; d++) {
     const int ntile = 4 - d;
@@ -175,89 +234,117 @@ __global__ __launch_bounds__(256) void k_chol_step(double *__restrict__ A, doubl
   __shared__ double PI[kNB * kLd];
   __shared__ double PJ[kNB * kLd];
   __shared__ double DV[kNB * kLd];
+  __shared__ double CT[kNB * kLd];
   __shared__ double Ys[3 * 256];
-  __shared__ double dsq[kNB];
+  __shared__ double wbuf[32];
   const int tid = threadIdx.x;
+  int fblk = 0;  // diagonal block factored by this workgroup (if any)
   if (b < 0) {
-    for (int e = tid; e < kNB * kNB; e += 256) {
-      const int r = e >> 6, c = e & 63;
-      DV[r * kLd + c] = (r < n && c < n) ? A[(int64_t)r * n + c] : (r == c ? 1.0 : 0.0);
+    double rv[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int e = tid + 256 * q, r = e >> 6, c = e & 63;
+      rv[q] = (r < n && c < n) ? A[(int64_t)r * n + c] : (r == c ? 1.0 : 0.0);
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int e = tid + 256 * q;
+      DV[(e >> 6) * kLd + (e & 63)] = rv[q];
+    }
+  } else {
+    int p = blockIdx.x, I = b + 1;
+    while (p >= I - b) {
+      p -= I - b;
+      I++;
+    }
+    const int J = b + 1 + p;
+    const int k0 = b * kNB;
+    // stage the panel tiles P_I, P_J, L_bb^{-1} and the target tile C_IJ;
+    // all loads of a thread are issued before its LDS stores
+    {
+      double ri[16], rj[16], rd[16], rc[16];
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        const int e = tid + 256 * q, r = e >> 6, c = e & 63;
+        const int64_t gi = (int64_t)I * kNB + r, gj = (int64_t)J * kNB + r,
+                      gjc = (int64_t)J * kNB + c;
+        ri[q] = (gi < n) ? A[gi * n + k0 + c] : 0.0;
+        rj[q] = (J != I && gj < n) ? A[gj * n + k0 + c] : 0.0;
+        rd[q] = Dinv[(int64_t)b * kNB * kNB + e];
+        rc[q] = (gi < n && gjc < n) ? A[gi * n + gjc] : (r == c ? 1.0 : 0.0);
+      }
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        const int e = tid + 256 * q, o = (e >> 6) * kLd + (e & 63);
+        PI[o] = ri[q];
+        if (J != I) PJ[o] = rj[q];
+        DV[o] = rd[q];
+        CT[o] = rc[q];
+      }
     }
     __syncthreads();
-    factor_tile(DV, PJ, Ys, dsq, info);
-    store_diag(DV, PJ, Lf, Dinv, n, 0);
-    return;
-  }
-  int p = blockIdx.x, I = b + 1;
-  while (p >= I - b) {
-    p -= I - b;
-    I++;
-  }
-  const int J = b + 1 + p;
-  const int k0 = b * kNB;
-  for (int e = tid; e < kNB * kNB; e += 256) {
-    const int r = e >> 6, c = e & 63;
-    const int64_t gi = (int64_t)I * kNB + r, gj = (int64_t)J * kNB + r;
-    PI[r * kLd + c] = (gi < n) ? A[gi * n + k0 + c] : 0.0;
-    if (J != I) PJ[r * kLd + c] = (gj < n) ? A[gj * n + k0 + c] : 0.0;
-    DV[r * kLd + c] = Dinv[(int64_t)b * kNB * kNB + e];
-  }
-  __syncthreads();
-  const int w = tid >> 6, lane = tid & 63;
-  const int qr = (w >> 1) * 32, qc = (w & 1) * 32;
-  double4_t qi[2][2], qj[2][2];
-  zero_acc(qi);
-  zero_acc(qj);
-  mm_nt_tile(PI, DV, qr, qc, lane, qi);  // Q_I = P_I L_bb^{-T}
-  if (J != I) mm_nt_tile(PJ, DV, qr, qc, lane, qj);
-  __syncthreads();
-  store_acc(PI, qi, qr, qc, lane);
-  if (J != I) store_acc(PJ, qj, qr, qc, lane);
-  __syncthreads();
-  if (I == J) {
-    for (int e = tid; e < kNB * kNB; e += 256) {
-      const int r = e >> 6, c = e & 63;
-      const int64_t gi = (int64_t)I * kNB + r;
-      if (gi < n) Lf[gi * n + k0 + c] = PI[r * kLd + c];
-    }
-  }
-  double4_t cacc[2][2];
-  zero_acc(cacc);
-  mm_nt_tile(PI, (J != I) ? PJ : PI, qr, qc, lane, cacc);  // Q_I Q_J^T
-  const bool next_diag = (I == b + 1);  // then J == I as well
-  const int li = lane & 15, lk = lane >> 4;
-#pragma unroll
-  for (int x = 0; x < 2; x++)
-#pragma unroll
-    for (int y = 0; y < 2; y++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int r = qr + 16 * x + lk + 4 * v, c = qc + 16 * y + li;
-        const int64_t gi = (int64_t)I * kNB + r, gj = (int64_t)J * kNB + c;
-        const bool in = gi < n && gj < n;
-        const double a = in ? A[gi * n + gj] - cacc[x][y][v] : (r == c ? 1.0 : 0.0);
-        if (next_diag) DV[r * kLd + c] = a;
-        else if (in) A[gi * n + gj] = a;
+    const int w = tid >> 6, lane = tid & 63;
+    const int qr = (w >> 1) * 32, qc = (w & 1) * 32;
+    double4_t qi[2][2], qj[2][2];
+    zero_acc(qi);
+    zero_acc(qj);
+    mm_nt_tile(PI, DV, qr, qc, lane, qi);  // Q_I = P_I L_bb^{-T}
+    if (J != I) mm_nt_tile(PJ, DV, qr, qc, lane, qj);
+    __syncthreads();
+    store_acc(PI, qi, qr, qc, lane);
+    if (J != I) store_acc(PJ, qj, qr, qc, lane);
+    __syncthreads();
+    if (I == J) {
+#pragma unroll 4
+      for (int e = tid; e < kNB * kNB; e += 256) {
+        const int r = e >> 6, c = e & 63;
+        const int64_t gi = (int64_t)I * kNB + r;
+        if (gi < n) Lf[gi * n + k0 + c] = PI[r * kLd + c];
       }
-  if (!next_diag) return;
+    }
+    double4_t cacc[2][2];
+    zero_acc(cacc);
+    mm_nt_tile(PI, (J != I) ? PJ : PI, qr, qc, lane, cacc);  // Q_I Q_J^T
+    const bool next_diag = (I == b + 1);  // then J == I as well
+    const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+      for (int y = 0; y < 2; y++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int r = qr + 16 * x + lk + 4 * v, c = qc + 16 * y + li;
+          const int64_t gi = (int64_t)I * kNB + r, gj = (int64_t)J * kNB + c;
+          const double a = CT[r * kLd + c] - cacc[x][y][v];
+          if (next_diag) DV[r * kLd + c] = (gi < n && gj < n) ? a : (r == c ? 1.0 : 0.0);
+          else if (gi < n && gj < n) A[gi * n + gj] = a;
+        }
+    if (!next_diag) return;
+    fblk = b + 1;
+  }
   __syncthreads();
-  factor_tile(DV, PJ, Ys, dsq, info);
-  store_diag(DV, PJ, Lf, Dinv, n, b + 1);
+  factor_tile(DV, PJ, Ys, wbuf, info);
+  store_diag(DV, PJ, Lf, Dinv, n, fblk);
 }
 
 // ---------------------------------------------------------------------------
 // W = L^{-1}, WT = W^T.  Workgroup s owns columns [16s, 16s+16), starting at
 // block row J = 16s/64 (W_J = columns of L_JJ^{-1}); below it
 //   W_I = -L_II^{-1} sum_{K=max(J, I-kband)}^{I-1} L_IK W_K,
-// with the last kband+1 block rows of the strip kept in an LDS ring.
+// with the last kband+1 block rows of the strip kept in an LDS ring (for
+// bands too wide for LDS, nring = 0, they are read back from W).  The
+// L_IK tiles are staged through LDS with every thread's loads of a tile in
+// flight together, the next tile loading while the current one is used;
+// the rows of L_II^{-1} a wave needs are loaded at the start of the step.
 __global__ __launch_bounds__(256) void k_linv_strip(const double *__restrict__ Lf,
                                                     const double *__restrict__ Dinv, int n,
                                                     int kband, int nring,
                                                     double *__restrict__ W,
                                                     double *__restrict__ WT) {
   extern __shared__ double lds[];
-  double *ring = lds;                          // [nring][64][16]
-  double *Bx = lds + (size_t)nring * kNB * kStrip;  // [64][17]
+  double *ring = lds;                               // [nring][64][16]
+  double *Bx = ring + (size_t)nring * kNB * kStrip;  // [64][17]
+  double *Lt = Bx + kNB * 17;                        // [2][64][kLd]
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const int li = lane & 15, lk = lane >> 4;
   const int nblk = (n + kNB - 1) / kNB;
@@ -265,11 +352,11 @@ __global__ __launch_bounds__(256) void k_linv_strip(const double *__restrict__ L
   const int J = c0 / kNB, cofs = c0 - J * kNB;
   const int gc = c0 + li;
   {
-    double *slot = ring + (size_t)(J % nring) * kNB * kStrip;
+    double *slot = nring ? ring + (size_t)(J % nring) * kNB * kStrip : nullptr;
     for (int e = tid; e < kNB * kStrip; e += 256) {
       const int r = e >> 4, j = e & 15;
       const double v = Dinv[(int64_t)J * kNB * kNB + r * kNB + cofs + j];
-      slot[e] = v;
+      if (nring) slot[e] = v;
       const int64_t gr = (int64_t)J * kNB + r, gcol = c0 + j;
       if (gr < n && gcol < n) {
         W[gr * n + gcol] = v;
@@ -277,33 +364,73 @@ __global__ __launch_bounds__(256) void k_linv_strip(const double *__restrict__ L
       }
     }
   }
+  auto load_tile = [&](int I, int K, double (&reg)[16]) {
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int e = tid + 256 * q, r = e >> 6, c = e & 63;
+      const int64_t row = (int64_t)I * kNB + r;
+      reg[q] = row < n ? Lf[row * n + (int64_t)K * kNB + c] : 0.0;
+    }
+  };
+  auto store_tile = [&](double *dst, const double (&reg)[16]) {
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int e = tid + 256 * q;
+      dst[(e >> 6) * kLd + (e & 63)] = reg[q];
+    }
+  };
   __syncthreads();
   for (int I = J + 1; I < nblk; I++) {
     const int Kmin = max(J, I - kband);
-    double4_t acc = {0.0, 0.0, 0.0, 0.0};
-    const int64_t row = (int64_t)I * kNB + 16 * w + li;
-    const double *lrow = Lf + (row < n ? row : 0) * (int64_t)n;
+    // rows 16w.. of L_II^{-1}: columns k < 16(w+1) are non-zero
+    const double *D = Dinv + (int64_t)I * kNB * kNB;
+    double dreg[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+      dreg[q] = (4 * q < 16 * (w + 1)) ? D[(16 * w + li) * kNB + 4 * q + lk] : 0.0;
+    double treg[16];
+    load_tile(I, Kmin, treg);
+    store_tile(Lt, treg);
+    __syncthreads();
+    double4_t acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
     for (int K = Kmin; K < I; K++) {
-      const double *slot = ring + (size_t)(K % nring) * kNB * kStrip;
-#pragma unroll 4
-      for (int k = 0; k < kNB; k += 4) {
-        const double a = row < n ? lrow[K * kNB + k + lk] : 0.0;
-        acc = mfma4(a, slot[(k + lk) * kStrip + li], acc);
+      const double *cur = Lt + (size_t)((K - Kmin) & 1) * kNB * kLd;
+      double *nxt = Lt + (size_t)(((K - Kmin) & 1) ^ 1) * kNB * kLd;
+      if (K + 1 < I) load_tile(I, K + 1, treg);
+      if (nring) {
+        const double *slot = ring + (size_t)(K % nring) * kNB * kStrip;
+#pragma unroll
+        for (int k = 0; k < kNB; k += 8) {
+          acc0 = mfma4(cur[(16 * w + li) * kLd + k + lk], slot[(k + lk) * kStrip + li], acc0);
+          acc1 = mfma4(cur[(16 * w + li) * kLd + k + 4 + lk], slot[(k + 4 + lk) * kStrip + li],
+                       acc1);
+        }
+      } else {
+        const double *wk = W + (int64_t)K * kNB * n + gc;
+#pragma unroll
+        for (int k = 0; k < kNB; k += 8) {
+          const double b0 = gc < n ? wk[(int64_t)(k + lk) * n] : 0.0;
+          const double b1 = gc < n ? wk[(int64_t)(k + 4 + lk) * n] : 0.0;
+          acc0 = mfma4(cur[(16 * w + li) * kLd + k + lk], b0, acc0);
+          acc1 = mfma4(cur[(16 * w + li) * kLd + k + 4 + lk], b1, acc1);
+        }
       }
+      if (K + 1 < I) store_tile(nxt, treg);
+      __syncthreads();
     }
 #pragma unroll
-    for (int v = 0; v < 4; v++) Bx[(16 * w + lk + 4 * v) * 17 + li] = acc[v];
+    for (int v = 0; v < 4; v++) Bx[(16 * w + lk + 4 * v) * 17 + li] = acc0[v] + acc1[v];
     __syncthreads();
     double4_t o = {0.0, 0.0, 0.0, 0.0};
-    const double *D = Dinv + (int64_t)I * kNB * kNB;
-    for (int k = 0; k < 16 * (w + 1); k += 4)
-      o = mfma4(D[(16 * w + li) * kNB + k + lk], Bx[(k + lk) * 17 + li], o);
-    double *slot = ring + (size_t)(I % nring) * kNB * kStrip;
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+      if (4 * q < 16 * (w + 1)) o = mfma4(dreg[q], Bx[(4 * q + lk) * 17 + li], o);
+    double *slot = nring ? ring + (size_t)(I % nring) * kNB * kStrip : nullptr;
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const int r = 16 * w + lk + 4 * v;
       const double val = -o[v];
-      slot[r * kStrip + li] = val;
+      if (nring) slot[r * kStrip + li] = val;
       const int64_t gr = (int64_t)I * kNB + r;
       if (gr < n && gc < n) {
         W[gr * n + gc] = val;
@@ -365,7 +492,7 @@ __global__ __launch_bounds__(256) void k_gemm_wtw(const double *__restrict__ WT,
 // ---------------------------------------------------------------------------
 // host side
 static size_t strip_lds_bytes(int nring) {
-  return sizeof(double) * ((size_t)nring * kNB * kStrip + (size_t)kNB * 17);
+  return sizeof(double) * ((size_t)nring * kNB * kStrip + (size_t)kNB * 17 + 2 * (size_t)kNB * kLd);
 }
 
 static void *zalloc(size_t bytes) { return dalloc(bytes); }
@@ -377,10 +504,7 @@ bool CoarseFactor::allocate(int n_, int bw_, bool f32) {
   nblk = (n + kNB - 1) / kNB;
   kband = (bw + kNB - 1) / kNB;
   nring = std::min(kband + 1, nblk);
-  if (strip_lds_bytes(nring) > 150 * 1024) {
-    set_error(TOP3D_ERR_ARG, "coarse solver: half bandwidth %d too wide (n=%d)", bw, n);
-    return false;
-  }
+  if (strip_lds_bytes(nring) > 158 * 1024) nring = 0;  // band rows read back from W
   const size_t nn = (size_t)n * n;
   A = (double *)zalloc(sizeof(double) * nn);
   L = (double *)zalloc(sizeof(double) * nn);
@@ -421,7 +545,7 @@ int CoarseFactor::factor() {
   static bool attr = false;
   if (!attr) {
     hipFuncSetAttribute((const void *)k_linv_strip, hipFuncAttributeMaxDynamicSharedMemorySize,
-                        150 * 1024);
+                        158 * 1024);
     attr = true;
   }
   hipLaunchKernelGGL(k_linv_strip, dim3((n + kStrip - 1) / kStrip), dim3(256), lds, s, L, Dinv,

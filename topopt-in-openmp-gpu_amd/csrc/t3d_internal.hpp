@@ -60,6 +60,7 @@ struct DevGrid {
   int64_t *fixed[kMaxLevels] = {};     // fixed DOF indices (device)
   double *KE[2] = {};                  // device KE_0 (576), KE_1 (8*576)
   float *KEf[2] = {};
+  double *KE2 = nullptr;               // device KE_2 (64*576), nl >= 3
   void *scratch[8] = {};               // lazily allocated work arrays of the API
   size_t scratch_bytes[8] = {};
   LevelArgs args(int l) const;
