@@ -74,7 +74,7 @@ __device__ inline void store_acc(double *X, const double4_t acc[2][2], int qr, i
 // ---------------------------------------------------------------------------
 // Cholesky and inverse of one 64x64 SPD tile in LDS, 256 threads (4 waves),
 // blocked by 16: per diagonal block kb
-//   (a) wave 0, lanes 0..15 (one row each, broadcasts through LDS): the 16x16
+//   (a) wave 0, lanes 0..15 (one row each, cross-lane reads): the 16x16
 //       Cholesky L_D and its inverse X_D;
 //   (b) the panel below: P <- P X_D^T;
 //   (c) the trailing lower triangle: C -= P P^T as 16x16 MFMA tiles;
@@ -82,12 +82,12 @@ __device__ inline void store_acc(double *X, const double4_t acc[2][2], int qr, i
 // In: T (lower triangle; rows/cols beyond the matrix padded with identity).
 // Out: T = L (zeros above the diagonal), X = L^{-1}.  A non-positive pivot
 // sets *info (and is replaced by 1 so that the factorization finishes).
-// Orders the LDS accesses of one wavefront (lanes of a wave execute their LDS
-// instructions in order; the fences keep the compiler from moving them).
-__device__ inline void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+// Value of v in lane `lane` (a wave-uniform index), for every lane.
+__device__ inline double readlane_d(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
 __device__ __forceinline__ void factor_tile(double *T, double *X, double *Ys, double *wbuf,
@@ -98,42 +98,41 @@ __device__ __forceinline__ void factor_tile(double *T, double *X, double *Ys, do
   __syncthreads();
   for (int kb = 0; kb < 4; kb++) {
     const int d0 = 16 * kb;
-    if (t < 16) {  // (a): one row per lane, broadcasts through LDS
+    if (t < 16) {  // (a): one row per lane, broadcasts by v_readlane
       const int r = t;
       double v[16];
 #pragma unroll
       for (int c = 0; c < 16; c++) v[c] = (c <= r) ? T[(d0 + r) * kLd + d0 + c] : 0.0;
 #pragma unroll
       for (int j = 0; j < 16; j++) {
-        if (r == j) wbuf[16] = v[j];
-        wave_lds_sync();
-        double piv = wbuf[16];
+        double piv = readlane_d(v[j], j);
         if (!(piv > 0.0)) {
           if (r == 0) info[0] = 1;
           piv = 1.0;
         }
         const double ljj = sqrt(piv);
-        const double lrj = (r == j) ? ljj : v[j] / ljj;
+        const double inv = 1.0 / ljj;
+        const double lrj = (r == j) ? ljj : v[j] * inv;
         v[j] = (r >= j) ? lrj : 0.0;
-        wbuf[r] = lrj;
-        wave_lds_sync();
 #pragma unroll
-        for (int c = j + 1; c < 16; c++)
-          if (c <= r) v[c] -= lrj * wbuf[c];
-        wave_lds_sync();
+        for (int c = j + 1; c < 16; c++) {
+          const double lcj = readlane_d(lrj, c);
+          if (c <= r) v[c] -= lrj * lcj;
+        }
       }
 #pragma unroll
       for (int c = 0; c < 16; c++) T[(d0 + r) * kLd + d0 + c] = v[c];
-      wave_lds_sync();
-      // column r of X_D = L_D^{-1}: forward substitution, rows of L_D read
-      // by all lanes from LDS
-      double x[16];
+      // column r of X_D = L_D^{-1}: forward substitution with the rows of
+      // L_D broadcast from their lanes
+      double dinv[16], x[16];
+#pragma unroll
+      for (int i = 0; i < 16; i++) dinv[i] = 1.0 / readlane_d(v[i], i);
 #pragma unroll
       for (int i = 0; i < 16; i++) {
         double acc = (i == r) ? 1.0 : 0.0;
 #pragma unroll
-        for (int m = 0; m < i; m++) acc -= T[(d0 + i) * kLd + d0 + m] * x[m];
-        x[i] = acc / T[(d0 + i) * kLd + d0 + i];
+        for (int m = 0; m < i; m++) acc -= readlane_d(v[m], i) * x[m];
+        x[i] = acc * dinv[i];
         X[(d0 + i) * kLd + d0 + r] = x[i];
       }
     }

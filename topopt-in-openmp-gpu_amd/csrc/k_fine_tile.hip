@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "t3d_internal.hpp"
@@ -212,6 +213,23 @@ struct K1Coef {
   T v[kK1Nnz];  // non-zeros of K1, row-major over (3 sa + ca, 3 sb + cb)
 };
 
+// The kernel reads K1 from constant memory through an offset the compiler
+// cannot see to be zero, so that the 279 values are fetched by scalar loads
+// inside the element loop instead of being held (and spilled) in registers.
+__constant__ float c_k1f[kK1Nnz];
+__constant__ double c_k1d[kK1Nnz];
+
+__device__ __forceinline__ int opaque_zero() {
+  int z;
+  asm volatile("s_mov_b32 %0, 0" : "=s"(z));
+  return z;
+}
+
+template <typename T>
+__device__ __forceinline__ T k1coef(int off, int idx) {
+  if constexpr (sizeof(T) == 8) return c_k1d[off + idx]; else return c_k1f[off + idx];
+}
+
 // Host: K1 from the sub-cell matrices KE1 (8 x 576, sub-cell index
 // 4 ox + 2 oz + oy), checking the mirror relation for all 8 sub-cells and
 // the zero pattern.  Returns false if either fails.
@@ -261,10 +279,52 @@ bool level1_hadamard(const double *KE1, double *coef) {
   return true;
 }
 
+// One row (3 sa + ca) of the transformed product: sum over sigma groups sb
+// of Ehat[sa ^ sb] * sum_cb K1[row][(sb, cb)] H[sb][cb], expanded at compile
+// time over the non-zeros of kK1Mask.
+template <typename T, int ROW, int SB>
+__device__ __forceinline__ void l1_accum(T &acc, int koff, const T (&Eh)[8], const T (&H)[8][3]) {
+  constexpr uint32_t m = (kK1Mask[ROW] >> (3 * SB)) & 7u;
+  if constexpr (m != 0) {
+    constexpr int base =
+        k1_offset(ROW) + __builtin_popcount(kK1Mask[ROW] & ((1u << (3 * SB)) - 1u));
+    T part;
+    if constexpr ((m & 1u) != 0) {
+      part = k1coef<T>(koff, base) * H[SB][0];
+      if constexpr ((m & 2u) != 0) part += k1coef<T>(koff, base + 1) * H[SB][1];
+      if constexpr ((m & 4u) != 0)
+        part += k1coef<T>(koff, base + ((m & 2u) ? 2 : 1)) * H[SB][2];
+    } else if constexpr ((m & 2u) != 0) {
+      part = k1coef<T>(koff, base) * H[SB][1];
+      if constexpr ((m & 4u) != 0) part += k1coef<T>(koff, base + 1) * H[SB][2];
+    } else {
+      part = k1coef<T>(koff, base) * H[SB][2];
+    }
+    acc += Eh[(ROW / 3) ^ SB] * part;
+  }
+}
+
+template <typename T, int ROW, int... SB>
+__device__ __forceinline__ T l1_row(int koff, const T (&Eh)[8], const T (&H)[8][3],
+                                    std::integer_sequence<int, SB...>) {
+  T acc = 0;
+  (l1_accum<T, ROW, SB>(acc, koff, Eh, H), ...);
+  return acc;
+}
+
+template <typename T, int... ROW>
+__device__ __forceinline__ void l1_rows(T (&g)[8][3], int koff, const T (&Eh)[8],
+                                        const T (&H)[8][3], std::integer_sequence<int, ROW...>) {
+  ((g[(ROW / 3) ^ (1 << (ROW % 3))][ROW % 3] =
+        l1_row<T, ROW>(koff, Eh, H, std::make_integer_sequence<int, 8>{})),
+   ...);
+}
+
 // y = (sum_o E_o KE1[o]) u for one level-1 element, Eh = transformed moduli
 template <typename T>
-__device__ __forceinline__ void element_apply_l1(const K1Coef<T> &K, const T (&Eh)[8],
-                                                 const T (&u)[2][2][2][3], T (&y)[2][2][2][3]) {
+__device__ __forceinline__ void element_apply_l1(const T (&Eh)[8], const T (&u)[2][2][2][3],
+                                                 T (&y)[2][2][2][3]) {
+  const int koff = opaque_zero();
   T h[8][3];
   wht_fwd<T>(u, h);
   T H[8][3];  // relabelled: H[sigma][c] = h[sigma ^ (1 << c)][c]
@@ -273,27 +333,7 @@ __device__ __forceinline__ void element_apply_l1(const K1Coef<T> &K, const T (&E
 #pragma unroll
     for (int c = 0; c < 3; c++) H[sg][c] = h[sg ^ (1 << c)][c];
   T g[8][3];
-#pragma unroll
-  for (int sa = 0; sa < 8; sa++)
-#pragma unroll
-    for (int ca = 0; ca < 3; ca++) {
-      const int row = 3 * sa + ca;
-      T acc = 0;
-#pragma unroll
-      for (int sb = 0; sb < 8; sb++) {
-        if (((kK1Mask[row] >> (3 * sb)) & 7) == 0) continue;
-        T part = 0;
-#pragma unroll
-        for (int cb = 0; cb < 3; cb++) {
-          const int col = 3 * sb + cb;
-          if ((kK1Mask[row] >> col) & 1)
-            part += K.v[k1_offset(row) + __builtin_popcount(kK1Mask[row] & ((1u << col) - 1u))] *
-                    H[sb][cb];
-        }
-        acc += Eh[sa ^ sb] * part;
-      }
-      g[sa ^ (1 << ca)][ca] = acc;
-    }
+  l1_rows<T>(g, koff, Eh, H, std::make_integer_sequence<int, 24>{});
   wht_inv<T>(g, y);
 }
 
@@ -309,7 +349,7 @@ struct TileArgs {
 // moduli per element.  MODE: apply / Jacobi sweep / residual.
 template <typename T, int LVL, int MODE, bool BCSYM>
 __global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_tile_sweep(
-    TileArgs a, std::conditional_t<LVL == 0, Blocks<T>, K1Coef<T>> B, const T *__restrict__ E,
+    TileArgs a, std::conditional_t<LVL == 0, Blocks<T>, int> B, const T *__restrict__ E,
     const T *__restrict__ in, T *__restrict__ out, const T *__restrict__ b,
     const T *__restrict__ invD, T omega, const uint8_t *__restrict__ fix) {
   extern __shared__ unsigned char smem_raw[];
@@ -416,7 +456,7 @@ __global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_tile_sweep(
           Eh[q + 4] = p2[q] - p2[q + 4];
         }
       }
-      element_apply_l1<T>(B, Eh, u, y);
+      element_apply_l1<T>(Eh, u, y);
     }
     if (i >= i0) {
       // plane i: x- face of slab i plus the pending x+ face of slab i-1
@@ -520,6 +560,17 @@ static void init_coef() {
     }
 }
 
+// K1 into this device's constant memory (once per device)
+static void upload_k1() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return;
+  static uint64_t done = 0;
+  if ((done >> dev) & 1) return;
+  if (T3D_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_k1f), g_coef.k32.v, sizeof(g_coef.k32.v))) &&
+      T3D_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_k1d), g_coef.k64.v, sizeof(g_coef.k64.v))))
+    done |= 1ull << dev;
+}
+
 bool tile_level1_available() {
   init_coef();
   return g_coef.ok1;
@@ -602,11 +653,12 @@ static void launch_tile(const DevGrid &g, int mode, const T *E, const T *in, T *
   const int bs = (nthr + 63) / 64 * 64;
   const size_t shm = sizeof(T) * 2 * 9 * (size_t)nthr;
   dim3 grid(s.ntx, s.nty * s.ntz);
-  std::conditional_t<LVL == 0, Blocks<T>, K1Coef<T>> B;
+  std::conditional_t<LVL == 0, Blocks<T>, int> B;
   if constexpr (LVL == 0) {
     if constexpr (sizeof(T) == 8) B = g_coef.b64; else B = g_coef.b32;
   } else {
-    if constexpr (sizeof(T) == 8) B = g_coef.k64; else B = g_coef.k32;
+    B = 0;
+    upload_k1();
   }
 #define T3D_LAUNCH(M, S)                                                                    \
   hipLaunchKernelGGL((k_tile_sweep<T, LVL, M, S>), grid, dim3(bs), shm, stream(), a, B, E, \
