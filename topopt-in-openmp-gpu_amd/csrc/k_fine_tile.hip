@@ -19,6 +19,7 @@
 // owners through LDS (9 values, one barrier).  Tiles overlap by one element
 // in every direction, so every owned node of a tile interior is complete
 // without atomics and the result is deterministic.
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -583,24 +584,27 @@ struct TileShape {
 // Tile of TY x TZ element columns marching over NX node planes.  LDS holds
 // two exchange buffers of 9 values per thread (under 60 KiB).  Tuning
 // override: TOP3D_TILE_L<level>=TY,TZ,NX.
-static TileShape tile_shape(const Level &L, int lvl, int elem_bytes) {
-  TileShape s;
+//
+// Shapes measured on MI355X at 256x128x128 (fine) / 128x64x64 (level 1):
+// small workgroups (200-300 threads) for the fine level, so that several
+// share a CU and their barriers interleave; wide, short x-marches for level
+// 1.  The x-march length follows from the number of workgroups the device
+// holds at once (`slots`): a launch that overflows them by a few
+// workgroups pays a whole extra round, so NX is chosen to minimise
+// rounds x (NX + 3), the per-workgroup cost in slab steps.
+static TileShape tile_shape(const Level &L, int lvl, int elem_bytes, int &nx_fixed) {
+  TileShape s{};
   const int lds_threads = 60 * 1024 / (18 * elem_bytes);
   const int max_threads = std::min(elem_bytes == 4 ? 1024 : 512, lds_threads);
-  // Shapes measured on MI355X at 256x128x128 (fine) / 128x64x64 (level 1):
-  // small workgroups (200-300 threads) for the fine level, so that several
-  // share a CU and their barriers interleave; wider, short x-marches for
-  // level 1.  ycap/tz bound the tile, nx fixes the x-march (0: from the
-  // workgroup target).
   struct Rule {
-    int ycap, tz, nx, wg_target;
+    int ycap, tz, nx;
   };
-  static constexpr Rule kRules[2][2] = {{{33, 6, 0, 2000}, {33, 8, 0, 1000}},   // level 0: f64, f32
-                                        {{65, 6, 8, 0}, {65, 12, 4, 0}}};        // level 1
+  static constexpr Rule kRules[2][2] = {{{33, 6, 0}, {33, 8, 0}},   // level 0: f64, f32
+                                        {{65, 6, 8}, {65, 12, 4}}};  // level 1
   const Rule r = kRules[lvl == 0 ? 0 : 1][elem_bytes == 4 ? 1 : 0];
   s.TY = std::max(2, std::min(L.nely + 1, r.ycap));
   s.TZ = std::max(2, std::min({L.nelz + 1, r.tz, max_threads / s.TY}));
-  int nx_fixed = r.nx;
+  nx_fixed = r.nx;
   const char *env = std::getenv(lvl == 0 ? "TOP3D_TILE_L0" : "TOP3D_TILE_L1");
   int oy = 0, oz = 0, ox = 0;
   if (env && std::sscanf(env, "%d,%d,%d", &oy, &oz, &ox) == 3 && oy >= 2 && oz >= 2 && ox >= 1 &&
@@ -614,17 +618,55 @@ static TileShape tile_shape(const Level &L, int lvl, int elem_bytes) {
   auto tiles = [](int nodes, int T) { return nodes <= T ? 1 : (nodes - T + T - 2) / (T - 1) + 1; };
   s.nty = tiles(L.ny, s.TY);
   s.ntz = tiles(L.nz, s.TZ);
+  return s;
+}
+
+static void tile_x(TileShape &s, const Level &L, int nx_fixed, int64_t slots) {
   const int64_t yz = (int64_t)s.nty * s.ntz;
   if (nx_fixed > 0) {
     s.NX = std::min(nx_fixed, L.nx);
   } else {
-    int ntx = (int)std::max<int64_t>(1, (r.wg_target + yz - 1) / yz);
-    ntx = std::min(ntx, L.nx);
-    s.NX = std::max(4, (L.nx + ntx - 1) / ntx);
-    s.NX = std::min(s.NX, L.nx);
+    int best = L.nx;
+    int64_t best_cost = INT64_MAX;
+    for (int k = 1; k <= 8; k++) {
+      const int64_t ntx = std::max<int64_t>(1, std::min<int64_t>(L.nx, k * slots / yz));
+      const int nx = (int)((L.nx + ntx - 1) / ntx);
+      const int64_t rounds = (((L.nx + nx - 1) / nx) * yz + slots - 1) / slots;
+      const int64_t cost = rounds * (nx + 3);
+      if (cost < best_cost) {
+        best_cost = cost;
+        best = nx;
+      }
+    }
+    s.NX = best;
   }
   s.ntx = (L.nx + s.NX - 1) / s.NX;
-  return s;
+}
+
+// workgroups of `nthr` threads and `shm` bytes of LDS one CU holds at once
+static int wg_per_cu(const void *kernel, int nthr, size_t shm) {
+  hipFuncAttributes at{};
+  int per_simd = 4;
+  if (hipFuncGetAttributes(&at, kernel) == hipSuccess && at.numRegs > 0)
+    per_simd = std::max(1, std::min(8, 512 / ((at.numRegs + 7) / 8 * 8)));
+  const int waves = (nthr + 63) / 64;
+  const int by_waves = 4 * per_simd / waves;
+  const int by_lds = (int)((160 * 1024) / std::max<size_t>(shm, 1));
+  return std::max(1, std::min(by_waves, by_lds));
+}
+
+static int device_cus() {
+  static int ncu[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!ncu[dev]) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        v <= 0)
+      v = 256;
+    ncu[dev] = v;
+  }
+  return ncu[dev];
 }
 
 template <typename T, int LVL>
@@ -636,7 +678,17 @@ static void launch_tile(const DevGrid &g, int mode, const T *E, const T *in, T *
     return;
   }
   const Level &L = g.lv[LVL];
-  const TileShape s = tile_shape(L, LVL, (int)sizeof(T));
+  int nx_fixed = 0;
+  TileShape s = tile_shape(L, LVL, (int)sizeof(T), nx_fixed);
+  const int nthr = s.TY * s.TZ;
+  const int bs = (nthr + 63) / 64 * 64;
+  const size_t shm = sizeof(T) * 2 * 9 * (size_t)nthr;
+  static int cached_nthr = -1, cached_slots = 1;  // per instantiation
+  if (cached_nthr != nthr) {
+    cached_slots = wg_per_cu((const void *)k_tile_sweep<T, LVL, kModeJacobi, false>, bs, shm);
+    cached_nthr = nthr;
+  }
+  tile_x(s, L, nx_fixed, (int64_t)cached_slots * device_cus());
   TileArgs a;
   a.nelx = L.nelx;
   a.nely = L.nely;
@@ -649,9 +701,6 @@ static void launch_tile(const DevGrid &g, int mode, const T *E, const T *in, T *
   a.TZ = s.TZ;
   a.NX = s.NX;
   a.nty = s.nty;
-  const int nthr = s.TY * s.TZ;
-  const int bs = (nthr + 63) / 64 * 64;
-  const size_t shm = sizeof(T) * 2 * 9 * (size_t)nthr;
   dim3 grid(s.ntx, s.nty * s.ntz);
   std::conditional_t<LVL == 0, Blocks<T>, int> B;
   if constexpr (LVL == 0) {
