@@ -100,6 +100,118 @@ void launch_axpy_cg(int64_t n, double *scal, double *u, double *r, const double 
                      r, p, q);
 }
 
+// ---- fused CG recurrence ---------------------------------------------------
+// The CG scalars are reduced from per-workgroup partial sums by every block
+// of the consuming kernel, in one fixed order, so no separate reduction
+// launch sits between the producer and the consumer.
+
+// block-wide sum returned to every thread
+__device__ __forceinline__ double block_sum_all(double v) {
+  __shared__ double sm[kRedBlock / 64 + 1];
+  __syncthreads();
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sm[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0;
+    for (int i = 0; i < (int)(blockDim.x + 63) / 64; i++) s += sm[i];
+    sm[kRedBlock / 64] = s;
+  }
+  __syncthreads();
+  return sm[kRedBlock / 64];
+}
+
+__device__ __forceinline__ double sum_partials_all(const double *__restrict__ partials, int n) {
+  double s = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += partials[i];
+  return block_sum_all(s);
+}
+
+// rho partials: r . z, z in the V-cycle's precision
+template <typename TZ>
+__global__ __launch_bounds__(kRedBlock) void k_dot_rz(const double *__restrict__ r,
+                                                      const TZ *__restrict__ z, int64_t n,
+                                                      double *__restrict__ partials) {
+  double s = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    s += r[i] * (double)z[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// rho = sum(partials); p = z (first) or p = (rho/rho_old) p + z
+// (stencil_solvers.c:350-362); block 0 stores s[0] = rho
+template <typename TZ>
+__global__ __launch_bounds__(kRedBlock) void k_pupdate_fused(
+    int64_t n, const double *__restrict__ partials, int npart, double *__restrict__ s,
+    double *__restrict__ p, const TZ *__restrict__ z, int first) {
+  const double rho = sum_partials_all(partials, npart);
+  if (blockIdx.x == 0 && threadIdx.x == 0) s[0] = rho;
+  const double beta = first ? 0.0 : rho / s[1];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double zi = (double)z[i];
+    p[i] = first ? zi : beta * p[i] + zi;
+  }
+}
+
+// pq = sum(partials); alpha = rho / pq; u += alpha p; r -= alpha q
+// (stencil_solvers.c:366-377); r32 = (float) r when given; r.r partials.
+// Block 0 stores rho_old = rho and pq.
+__global__ __launch_bounds__(kRedBlock) void k_axpy_fused(
+    int64_t n, const double *__restrict__ pq_part, int npart, double *__restrict__ s,
+    double *__restrict__ u, double *__restrict__ r, const double *__restrict__ p,
+    const double *__restrict__ q, float *__restrict__ r32, double *__restrict__ rr_part) {
+  const double pq = sum_partials_all(pq_part, npart);
+  const double rho = s[0];
+  const double alpha = rho / pq;
+  double acc = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    u[i] += alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    if (r32) r32[i] = (float)ri;
+    acc += ri * ri;
+  }
+  acc = block_sum_all(acc);
+  if (threadIdx.x == 0) {
+    rr_part[blockIdx.x] = acc;
+    if (blockIdx.x == 0) {
+      s[1] = rho;
+      s[2] = pq;
+    }
+  }
+}
+
+template <typename TZ>
+void launch_dot_rz(const double *r, const TZ *z, int64_t n, double *partials) {
+  hipLaunchKernelGGL((k_dot_rz<TZ>), dim3(kRedGrid), dim3(kRedBlock), 0, stream(), r, z, n,
+                     partials);
+}
+template <typename TZ>
+void launch_pupdate_fused(int64_t n, const double *partials, double *scal, double *p,
+                          const TZ *z, int first) {
+  hipLaunchKernelGGL((k_pupdate_fused<TZ>), dim3(kRedGrid), dim3(kRedBlock), 0, stream(), n,
+                     partials, kRedGrid, scal, p, z, first);
+}
+void launch_axpy_fused(int64_t n, const double *pq_part, int npart, double *scal, double *u,
+                       double *r, const double *p, const double *q, float *r32,
+                       double *rr_part, double *rr) {
+  hipLaunchKernelGGL(k_axpy_fused, dim3(kRedGrid), dim3(kRedBlock), 0, stream(), n, pq_part,
+                     npart, scal, u, r, p, q, r32, rr_part);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kRedBlock), 0, stream(), rr_part, kRedGrid,
+                     rr);
+}
+template void launch_dot_rz<double>(const double *, const double *, int64_t, double *);
+template void launch_dot_rz<float>(const double *, const float *, int64_t, double *);
+template void launch_pupdate_fused<double>(int64_t, const double *, double *, double *,
+                                           const double *, int);
+template void launch_pupdate_fused<float>(int64_t, const double *, double *, double *,
+                                          const float *, int);
+
 template <typename TO, typename TI>
 __global__ void k_convert(int64_t n, const TI *__restrict__ in, TO *__restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -352,7 +352,8 @@ template <typename T, int LVL, int MODE, bool BCSYM>
 __global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_tile_sweep(
     TileArgs a, std::conditional_t<LVL == 0, Blocks<T>, int> B, const T *__restrict__ E,
     const T *__restrict__ in, T *__restrict__ out, const T *__restrict__ b,
-    const T *__restrict__ invD, T omega, const uint8_t *__restrict__ fix) {
+    const T *__restrict__ invD, T omega, const uint8_t *__restrict__ fix,
+    double *__restrict__ dotp) {
   extern __shared__ unsigned char smem_raw[];
   T *xch = reinterpret_cast<T *>(smem_raw);  // [2][nthreads][9]
   const int nthr = a.TY * a.TZ;
@@ -378,6 +379,7 @@ __global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_tile_sweep(
   const int64_t noff[2][2] = {{0, a.wy}, {1, (int64_t)a.wy + 1}};  // [py][pz]
   T u[2][2][2][3];
   T pend[2][2][3];
+  double dacc = 0.0;  // apply mode with dotp: partial sum of in . out
 #pragma unroll
   for (int py = 0; py < 2; py++)
 #pragma unroll
@@ -507,6 +509,10 @@ __global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_tile_sweep(
           }
           if (MODE == kModeApply) {
             o = kv;
+            if (dotp) {
+              const T pv = (BCSYM && ((m >> c) & 1)) ? in[3 * nd + c] : uc;
+              dacc += (double)pv * (double)o;
+            }
           } else if (MODE == kModeJacobi) {
             const T uo = BCSYM ? in[3 * nd + c] : uc;
             o = uo + omega * invD[3 * nd + c] * (b[3 * nd + c] - kv);
@@ -523,6 +529,19 @@ __global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_tile_sweep(
       for (int pz = 0; pz < 2; pz++)
 #pragma unroll
         for (int c = 0; c < 3; c++) pend[py][pz][c] = y[1][py][pz][c];
+  }
+  if (MODE == kModeApply && dotp) {  // one partial per workgroup, fixed order
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) dacc += __shfl_down(dacc, off, 64);
+    __syncthreads();
+    double *red = reinterpret_cast<double *>(smem_raw);
+    if ((t & 63) == 0) red[t >> 6] = dacc;
+    __syncthreads();
+    if (t == 0) {
+      double s = 0.0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += red[w];
+      dotp[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = s;
+    }
   }
 }
 
@@ -670,12 +689,13 @@ static int device_cus() {
 }
 
 template <typename T, int LVL>
-static void launch_tile(const DevGrid &g, int mode, const T *E, const T *in, T *out, const T *b,
-                        const T *invD, T omega, int bcsym) {
+static int launch_tile(const DevGrid &g, int mode, const T *E, const T *in, T *out, const T *b,
+                       const T *invD, T omega, int bcsym, double *dotp = nullptr,
+                       int dot_cap = 0) {
   init_coef();
   if (!(LVL == 0 ? g_coef.ok0 : g_coef.ok1)) {
     set_error(TOP3D_ERR_ARG, "element matrix lacks the reflection structure");
-    return;
+    return 0;
   }
   const Level &L = g.lv[LVL];
   int nx_fixed = 0;
@@ -702,6 +722,8 @@ static void launch_tile(const DevGrid &g, int mode, const T *E, const T *in, T *
   a.NX = s.NX;
   a.nty = s.nty;
   dim3 grid(s.ntx, s.nty * s.ntz);
+  const int nwg = s.ntx * s.nty * s.ntz;
+  if (dotp && nwg > dot_cap) dotp = nullptr;
   std::conditional_t<LVL == 0, Blocks<T>, int> B;
   if constexpr (LVL == 0) {
     if constexpr (sizeof(T) == 8) B = g_coef.b64; else B = g_coef.b32;
@@ -711,7 +733,7 @@ static void launch_tile(const DevGrid &g, int mode, const T *E, const T *in, T *
   }
 #define T3D_LAUNCH(M, S)                                                                    \
   hipLaunchKernelGGL((k_tile_sweep<T, LVL, M, S>), grid, dim3(bs), shm, stream(), a, B, E, \
-                     in, out, b, invD, omega, g.fixmask[LVL])
+                     in, out, b, invD, omega, g.fixmask[LVL], dotp)
   if (bcsym) {
     if (mode == kModeApply) T3D_LAUNCH(kModeApply, true);
     else if (mode == kModeJacobi) T3D_LAUNCH(kModeJacobi, true);
@@ -722,12 +744,19 @@ static void launch_tile(const DevGrid &g, int mode, const T *E, const T *in, T *
     else T3D_LAUNCH(kModeResidual, false);
   }
 #undef T3D_LAUNCH
+  return dotp ? nwg : 0;
 }
 
 template <typename T>
 void launch_fine_tile(const DevGrid &g, int mode, const T *E, const T *in, T *out,
                       const T *b, const T *invD, T omega, int bcsym) {
   launch_tile<T, 0>(g, mode, E, in, out, b, invD, omega, bcsym);
+}
+
+int launch_fine_apply_dot(const DevGrid &g, const double *E, const double *p, double *q,
+                          int bcsym, double *partials, int cap) {
+  return launch_tile<double, 0>(g, kModeApply, E, p, q, nullptr, nullptr, 0.0, bcsym, partials,
+                                cap);
 }
 
 template <typename T>

@@ -75,6 +75,7 @@ bool SolverState::allocate(DevGrid *grid, int nlev) {
   if (!coarse.allocate(ncoarse, bandwidth, true)) return false;
   scal = alloc_t<double>(16);
   partials = alloc_t<double>(1024 + 16);
+  cgpart = alloc_t<double>(kPartRZ + kPartPQ + kPartRR);
   if (!T3D_HIP(hipHostMalloc((void **)&hscal, sizeof(double) * 16, hipHostMallocDefault)))
     return false;
   if (!T3D_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming))) return false;
@@ -99,7 +100,7 @@ void SolverState::release() {
   }
   f(r); f(p); f(q); f(z); f(E64); f(E32);
   coarse.release();
-  f(scal); f(partials);
+  f(scal); f(partials); f(cgpart);
   if (hscal) hipHostFree(hscal);
   if (ev) hipEventDestroy(ev);
   magic = 0;
@@ -188,15 +189,14 @@ void SolverState::vcycle(int l) {
     hipMemcpyAsync(z_, res, sizeof(T) * g->lv[l].ndof, hipMemcpyDeviceToDevice, stream());
 }
 
+// z = M r.  Mixed form: the V-cycle reads the FP32 copy of r written by the
+// CG update and leaves z in FP32, which the CG kernels read directly.
 void SolverState::precondition() {
   if (!mixed) {
     vcycle<double>(0);  // r -> z (aliases)
     return;
   }
-  const int64_t n0 = g->lv[0].ndof;
-  launch_convert<float, double>(n0, r, v32[0][kR]);
   vcycle<float>(0);
-  launch_convert<double, float>(n0, v32[0][kZ], z);
 }
 
 // ---------------------------------------------------------------------------
@@ -215,6 +215,7 @@ int SolverState::solve(const double *x, const double *b, double *u, double tol, 
   }
   launch_fine<double>(*g, 2, E64, z, r, b, nullptr, 0.0, bcsym);  // r = b - K z
   launch_dot<double>(b, b, n0, partials, scal + 4);
+  if (mixed) launch_convert<float, double>(n0, r, v32[0][kR]);
   hipMemcpyAsync(hscal + 4, scal + 4, sizeof(double), hipMemcpyDeviceToHost, stream());
   if (!T3D_HIP(hipStreamSynchronize(stream()))) return TOP3D_ERR_HIP;
   const auto t1 = clk::now();
@@ -224,15 +225,27 @@ int SolverState::solve(const double *x, const double *b, double *u, double tol, 
   int iter = 0;
   double relres = 0.0;
   bool pending = false;  // a V-cycle for iteration `iter` is already queued
+  double *part_rz = cgpart, *part_pq = cgpart + kPartRZ, *part_rr = part_pq + kPartPQ;
   for (iter = 0; iter < max_iter; iter++) {
     if (!pending) precondition();
     pending = false;
-    launch_dot<double>(r, z, n0, partials, scal + 0);
-    launch_pupdate(n0, scal, p, z, iter == 0);
-    launch_fine<double>(*g, 0, E64, p, q, nullptr, nullptr, 0.0, bcsym);
-    launch_dot<double>(p, q, n0, partials, scal + 2);
-    launch_axpy_cg(n0, scal, u, r, p, q);
-    launch_dot<double>(r, r, n0, partials, scal + 3);
+    // rho = r.z; p = z + (rho / rho_old) p
+    if (mixed) {
+      launch_dot_rz<float>(r, v32[0][kZ], n0, part_rz);
+      launch_pupdate_fused<float>(n0, part_rz, scal, p, v32[0][kZ], iter == 0);
+    } else {
+      launch_dot_rz<double>(r, z, n0, part_rz);
+      launch_pupdate_fused<double>(n0, part_rz, scal, p, z, iter == 0);
+    }
+    // q = K p with the partial sums of p.q
+    int npq = launch_fine_apply_dot(*g, E64, p, q, bcsym, part_pq, kPartPQ);
+    if (npq == 0) {
+      launch_dot_rz<double>(p, q, n0, part_pq);
+      npq = kPartRZ;
+    }
+    // u += alpha p, r -= alpha q (and its FP32 copy), r.r
+    launch_axpy_fused(n0, part_pq, npq, scal, u, r, p, q, mixed ? v32[0][kR] : nullptr,
+                      part_rr, scal + 3);
     hipMemcpyAsync(hscal + 3, scal + 3, sizeof(double), hipMemcpyDeviceToHost, stream());
     hipEventRecord(ev, stream());
     // queue the next V-cycle while the residual norm travels to the host; if

@@ -35,7 +35,9 @@ struct SolverState {
   int ncoarse = 0, bandwidth = 0;
   CoarseFactor coarse;
   // device scalars, reduction partials, pinned host copy
+  static constexpr int kPartRZ = 1024, kPartPQ = 1 << 16, kPartRR = 1024;
   double *scal = nullptr, *partials = nullptr, *hscal = nullptr;
+  double *cgpart = nullptr;  // partial sums of r.z, p.q, r.r
   hipEvent_t ev = nullptr;
   Top3dSolveStats stats{};
 

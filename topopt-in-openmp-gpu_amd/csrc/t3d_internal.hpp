@@ -179,6 +179,20 @@ template <typename T>
 void launch_add(int64_t n, const T *a, T *out);               // out += a
 void launch_pupdate(int64_t n, const double *scal, double *p, const double *z,
                     int first);
+// fused CG steps (partial sums consumed by the next kernel, kRedGrid = 1024
+// partials for launch_dot_rz and the r.r partials)
+template <typename TZ>
+void launch_dot_rz(const double *r, const TZ *z, int64_t n, double *partials);
+template <typename TZ>
+void launch_pupdate_fused(int64_t n, const double *partials, double *scal, double *p,
+                          const TZ *z, int first);
+void launch_axpy_fused(int64_t n, const double *pq_part, int npart, double *scal, double *u,
+                       double *r, const double *p, const double *q, float *r32,
+                       double *rr_part, double *rr);
+// FP64 fine apply q = K p with per-workgroup partials of p.q; returns their
+// count (0 if `cap` is too small)
+int launch_fine_apply_dot(const DevGrid &g, const double *E, const double *p, double *q,
+                          int bcsym, double *partials, int cap);
 void launch_copy_fixed(const DevGrid &g, int l, const double *in, double *out);
 
 // design-loop kernels
