@@ -90,78 +90,48 @@ struct Blocks {
   T m[8][3][3];
 };
 
-// forward per-component 8-point Walsh-Hadamard transform:
-// h[q][c] = sum_p (-1)^{q.p} u[p][c], q = qx | qy<<1 | qz<<2
+// The element transform h[q][c] = sum_p (-1)^{q.p} u[p][c] (p, q in {0,1}^3,
+// q = qx | qy<<1 | qz<<2) is split along x: with F_px[f][c] the 4-point
+// (y, z) transform of the element's face px (f = qy | qz<<1),
+//   h[2f] = F_0[f] + F_1[f],   h[2f+1] = F_0[f] - F_1[f].
+// A thread marching along x transforms each node plane once and reuses it
+// for the two elements that share it; the inverse runs the same way, and the
+// two element contributions to a plane are summed before its 4-point
+// inverse.
+
+// F[f][c] for the 4 nodes v[py][pz][c] of a face (its own inverse up to 4)
 template <typename T>
-__device__ __forceinline__ void wht_fwd(const T (&u)[2][2][2][3], T (&h)[8][3]) {
+__device__ __forceinline__ void face_wht(const T (&v)[2][2][3], T (&F)[4][3]) {
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    T a[2][2][2];
-#pragma unroll
-    for (int py = 0; py < 2; py++)
-#pragma unroll
-      for (int pz = 0; pz < 2; pz++) {
-        a[0][py][pz] = u[0][py][pz][c] + u[1][py][pz][c];
-        a[1][py][pz] = u[0][py][pz][c] - u[1][py][pz][c];
-      }
-    T b2[2][2][2];
-#pragma unroll
-    for (int qx = 0; qx < 2; qx++)
-#pragma unroll
-      for (int pz = 0; pz < 2; pz++) {
-        b2[qx][0][pz] = a[qx][0][pz] + a[qx][1][pz];
-        b2[qx][1][pz] = a[qx][0][pz] - a[qx][1][pz];
-      }
-#pragma unroll
-    for (int qx = 0; qx < 2; qx++)
-#pragma unroll
-      for (int qy = 0; qy < 2; qy++) {
-        h[qx | (qy << 1)][c] = b2[qx][qy][0] + b2[qx][qy][1];
-        h[qx | (qy << 1) | 4][c] = b2[qx][qy][0] - b2[qx][qy][1];
-      }
+    const T a0 = v[0][0][c] + v[1][0][c], a1 = v[0][0][c] - v[1][0][c];
+    const T b0 = v[0][1][c] + v[1][1][c], b1 = v[0][1][c] - v[1][1][c];
+    F[0][c] = a0 + b0;
+    F[1][c] = a1 + b1;
+    F[2][c] = a0 - b0;
+    F[3][c] = a1 - b1;
   }
 }
 
-// inverse (transpose) transform: y[p][c] = sum_q (-1)^{q.p} g[q][c]
+// node values y[py][pz][c] = sum_f (-1)^{qy py + qz pz} G[f][c]
 template <typename T>
-__device__ __forceinline__ void wht_inv(const T (&g)[8][3], T (&y)[2][2][2][3]) {
+__device__ __forceinline__ void face_iwht(const T (&G)[4][3], T (&y)[2][2][3]) {
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    T a[2][2][2];  // [qx][qy][pz]
-#pragma unroll
-    for (int qx = 0; qx < 2; qx++)
-#pragma unroll
-      for (int qy = 0; qy < 2; qy++) {
-        const T lo = g[qx | (qy << 1)][c], hi = g[qx | (qy << 1) | 4][c];
-        a[qx][qy][0] = lo + hi;
-        a[qx][qy][1] = lo - hi;
-      }
-    T b2[2][2][2];  // [qx][py][pz]
-#pragma unroll
-    for (int qx = 0; qx < 2; qx++)
-#pragma unroll
-      for (int pz = 0; pz < 2; pz++) {
-        b2[qx][0][pz] = a[qx][0][pz] + a[qx][1][pz];
-        b2[qx][1][pz] = a[qx][0][pz] - a[qx][1][pz];
-      }
-#pragma unroll
-    for (int py = 0; py < 2; py++)
-#pragma unroll
-      for (int pz = 0; pz < 2; pz++) {
-        y[0][py][pz][c] = b2[0][py][pz] + b2[1][py][pz];
-        y[1][py][pz][c] = b2[0][py][pz] - b2[1][py][pz];
-      }
+    const T a0 = G[0][c] + G[2][c], a1 = G[0][c] - G[2][c];  // qz summed: pz = 0, 1
+    const T b0 = G[1][c] + G[3][c], b1 = G[1][c] - G[3][c];
+    y[0][0][c] = a0 + b0;
+    y[1][0][c] = a0 - b0;
+    y[0][1][c] = a1 + b1;
+    y[1][1][c] = a1 - b1;
   }
 }
 
-// y = E * KE u for one fine element; u, y indexed [px][py][pz][c]
+// g = E (T KE T^T / 64) h for one fine element (transformed in and out)
 template <typename T>
-__device__ __forceinline__ void element_apply(const Blocks<T> &B, T E, const T (&u)[2][2][2][3],
-                                              T (&y)[2][2][2][3]) {
-  T h[8][3];
-  wht_fwd<T>(u, h);
+__device__ __forceinline__ void element_core(const Blocks<T> &B, T E, const T (&h)[8][3],
+                                             T (&g)[8][3]) {
   // block products, modulus, zero pattern of the reflection blocks
-  T g[8][3];
 #pragma unroll
   for (int s = 0; s < 8; s++) {
     T x[3];
@@ -182,7 +152,6 @@ __device__ __forceinline__ void element_apply(const Blocks<T> &B, T E, const T (
       g[s ^ (1 << c)][c] = E * acc;
     }
   }
-  wht_inv<T>(g, y);
 }
 
 // ---------------------------------------------------------------------------
@@ -321,21 +290,18 @@ __device__ __forceinline__ void l1_rows(T (&g)[8][3], int koff, const T (&Eh)[8]
    ...);
 }
 
-// y = (sum_o E_o KE1[o]) u for one level-1 element, Eh = transformed moduli
+// g = (T (sum_o E_o KE1[o]) T^T / 64) h for one level-1 element, Eh the
+// transformed moduli
 template <typename T>
-__device__ __forceinline__ void element_apply_l1(const T (&Eh)[8], const T (&u)[2][2][2][3],
-                                                 T (&y)[2][2][2][3]) {
+__device__ __forceinline__ void element_core_l1(const T (&Eh)[8], const T (&h)[8][3],
+                                                T (&g)[8][3]) {
   const int koff = opaque_zero();
-  T h[8][3];
-  wht_fwd<T>(u, h);
   T H[8][3];  // relabelled: H[sigma][c] = h[sigma ^ (1 << c)][c]
 #pragma unroll
   for (int sg = 0; sg < 8; sg++)
 #pragma unroll
     for (int c = 0; c < 3; c++) H[sg][c] = h[sg ^ (1 << c)][c];
-  T g[8][3];
   l1_rows<T>(g, koff, Eh, H, std::make_integer_sequence<int, 24>{});
-  wht_inv<T>(g, y);
 }
 
 struct TileArgs {
@@ -349,7 +315,7 @@ struct TileArgs {
 // LVL 0: fine level, one modulus per element; LVL 1: level 1, eight fine
 // moduli per element.  MODE: apply / Jacobi sweep / residual.
 template <typename T, int LVL, int MODE, bool BCSYM>
-__global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_tile_sweep(
+__device__ __forceinline__ void tile_sweep_body(
     TileArgs a, std::conditional_t<LVL == 0, Blocks<T>, int> B, const T *__restrict__ E,
     const T *__restrict__ in, T *__restrict__ out, const T *__restrict__ b,
     const T *__restrict__ invD, T omega, const uint8_t *__restrict__ fix,
@@ -374,30 +340,27 @@ __global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_tile_sweep(
                       (!last_z || k + 1 == a.nelz + 1);
   // level 1: the element is real (its fine moduli exist)
   const bool real_yz = elem_ok && j >= 1 && k >= 1;
-  const int64_t sx = (int64_t)a.wy * a.wz;
-  const int64_t n00 = (int64_t)k * a.wy + j;  // node (j, k) within a plane
-  const int64_t noff[2][2] = {{0, a.wy}, {1, (int64_t)a.wy + 1}};  // [py][pz]
-  T u[2][2][2][3];
-  T pend[2][2][3];
+  // 32-bit offsets: the host checks 3 * nodes < 2^31
+  const uint32_t sx = (uint32_t)a.wy * (uint32_t)a.wz;
+  const uint32_t n00 = (uint32_t)k * a.wy + j;  // node (j, k) within a plane
+  const uint32_t noff[2][2] = {{0u, (uint32_t)a.wy}, {1u, (uint32_t)a.wy + 1u}};  // [py][pz]
   double dacc = 0.0;  // apply mode with dotp: partial sum of in . out
-#pragma unroll
-  for (int py = 0; py < 2; py++)
-#pragma unroll
-    for (int pz = 0; pz < 2; pz++)
-#pragma unroll
-      for (int c = 0; c < 3; c++) pend[py][pz][c] = 0;
 
-  auto load_plane = [&](int px, int i) {
+  // transformed face of node plane i (zero outside the grid) and the raw
+  // values of its node (j+1, k+1)
+  auto load_face = [&](int i, T (&F)[4][3], T (&u11)[3]) {
+    T v[2][2][3];
 #pragma unroll
     for (int py = 0; py < 2; py++)
 #pragma unroll
       for (int pz = 0; pz < 2; pz++) {
         T v0 = 0, v1 = 0, v2 = 0;
         if (elem_ok && i <= a.nelx + 2) {
-          const int64_t nd = i * sx + n00 + noff[py][pz];
-          v0 = in[3 * nd];
-          v1 = in[3 * nd + 1];
-          v2 = in[3 * nd + 2];
+          const uint32_t nd = (uint32_t)i * sx + n00 + noff[py][pz];
+          const T *pn = in + 3u * nd;
+          v0 = pn[0];
+          v1 = pn[1];
+          v2 = pn[2];
           if (BCSYM) {
             const uint8_t m = fix[nd];
             if (m & 1) v0 = 0;
@@ -405,26 +368,39 @@ __global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_tile_sweep(
             if (m & 4) v2 = 0;
           }
         }
-        u[px][py][pz][0] = v0;
-        u[px][py][pz][1] = v1;
-        u[px][py][pz][2] = v2;
+        v[py][pz][0] = v0;
+        v[py][pz][1] = v1;
+        v[py][pz][2] = v2;
       }
+    u11[0] = v[1][1][0];
+    u11[1] = v[1][1][1];
+    u11[2] = v[1][1][2];
+    face_wht<T>(v, F);
   };
 
-  load_plane(1, i0 - 1);
-  for (int i = i0 - 1; i <= i1; i++) {  // element slab i: planes i, i+1
+  T pendT[4][3];  // transformed x+ face of the previous slab
 #pragma unroll
-    for (int py = 0; py < 2; py++)
+  for (int f = 0; f < 4; f++)
 #pragma unroll
-      for (int pz = 0; pz < 2; pz++)
+    for (int c = 0; c < 3; c++) pendT[f][c] = 0;
+
+  // element slab i (node planes i, i+1): Fc/uc the transformed face and the
+  // owned raw values of plane i, Fn/un those of plane i+1 (loaded here)
+  auto step = [&](int i, const T (&Fc)[4][3], const T (&uc3)[3], T (&Fn)[4][3], T (&un3)[3]) {
+    load_face(i + 1, Fn, un3);
+    T h[8][3];
 #pragma unroll
-        for (int c = 0; c < 3; c++) u[0][py][pz][c] = u[1][py][pz][c];
-    load_plane(1, i + 1);
-    T y[2][2][2][3];
+    for (int f = 0; f < 4; f++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        h[2 * f][c] = Fc[f][c] + Fn[f][c];
+        h[2 * f + 1][c] = Fc[f][c] - Fn[f][c];
+      }
+    T g[8][3];
     if constexpr (LVL == 0) {
       T Ee = 0;
-      if (elem_ok && i <= a.nelx + 1) Ee = E[((int64_t)i * a.fez + k) * a.few + j];
-      element_apply<T>(B, Ee, u, y);
+      if (elem_ok && i <= a.nelx + 1) Ee = E[((uint32_t)i * a.fez + k) * a.few + j];
+      element_core<T>(B, Ee, h, g);
     } else {
       T Eo[8];
 #pragma unroll
@@ -432,8 +408,8 @@ __global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_tile_sweep(
       if (real_yz && i >= 1 && i <= a.nelx) {
 #pragma unroll
         for (int o = 0; o < 8; o++) {
-          const int64_t fe = ((int64_t)(2 * i - 1 + (o & 1)) * a.fez + (2 * k - 1 + (o >> 2))) *
-                                 a.few + (2 * j - 1 + ((o >> 1) & 1));
+          const uint32_t fe = ((uint32_t)(2 * i - 1 + (o & 1)) * a.fez + (2 * k - 1 + (o >> 2))) *
+                                  a.few + (2 * j - 1 + ((o >> 1) & 1));
           Eo[o] = E[fe];
         }
       }
@@ -459,17 +435,17 @@ __global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_tile_sweep(
           Eh[q + 4] = p2[q] - p2[q + 4];
         }
       }
-      element_apply_l1<T>(Eh, u, y);
+      element_core_l1<T>(Eh, h, g);
     }
     if (i >= i0) {
-      // plane i: x- face of slab i plus the pending x+ face of slab i-1
-      T tot[2][2][3];
+      // plane i: x- face of slab i plus the pending x+ face of slab i-1,
+      // summed in the transformed basis
+      T totT[4][3], tot[2][2][3];
 #pragma unroll
-      for (int py = 0; py < 2; py++)
+      for (int f = 0; f < 4; f++)
 #pragma unroll
-        for (int pz = 0; pz < 2; pz++)
-#pragma unroll
-          for (int c = 0; c < 3; c++) tot[py][pz][c] = y[0][py][pz][c] + pend[py][pz][c];
+        for (int c = 0; c < 3; c++) totT[f][c] = (g[2 * f][c] + g[2 * f + 1][c]) + pendT[f][c];
+      face_iwht<T>(totT, tot);
       T *buf = xch + (size_t)(i & 1) * nthr * 9;
       if (active) {
 #pragma unroll
@@ -494,41 +470,68 @@ __global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_tile_sweep(
 #pragma unroll
           for (int c = 0; c < 3; c++) kin[c] += buf[9 * (t + a.TY + 1) + 6 + c];
         }
-        const int jn = j + 1, kn = k + 1;
-        const int64_t nd = i * sx + (int64_t)kn * a.wy + jn;
+        const uint32_t nd = (uint32_t)i * sx + (uint32_t)(k + 1) * a.wy + (uint32_t)(j + 1);
+        const uint32_t d0 = 3u * nd;
+        const T *pin = in + d0;
         const uint8_t m = fix[nd];
+        T bb[3] = {0, 0, 0}, dd[3] = {0, 0, 0};
+        if (MODE != kModeApply) {
+          const T *pb = b + d0;
+          bb[0] = pb[0];
+          bb[1] = pb[1];
+          bb[2] = pb[2];
+        }
+        if (MODE == kModeJacobi) {
+          const T *pd = invD + d0;
+          dd[0] = pd[0];
+          dd[1] = pd[1];
+          dd[2] = pd[2];
+        }
+        T o[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-          const T uc = u[0][1][1][c];
+          const T uc = uc3[c];
           T kv = kin[c];
-          T o;
           if (BCSYM) {  // the loaded value is masked; the BC row needs in
-            if ((m >> c) & 1) kv = in[3 * nd + c];
+            if ((m >> c) & 1) kv = pin[c];
           } else {
             if ((m >> c) & 1) kv = uc;
           }
           if (MODE == kModeApply) {
-            o = kv;
+            o[c] = kv;
             if (dotp) {
-              const T pv = (BCSYM && ((m >> c) & 1)) ? in[3 * nd + c] : uc;
-              dacc += (double)pv * (double)o;
+              const T pv = (BCSYM && ((m >> c) & 1)) ? pin[c] : uc;
+              dacc += (double)pv * (double)o[c];
             }
           } else if (MODE == kModeJacobi) {
-            const T uo = BCSYM ? in[3 * nd + c] : uc;
-            o = uo + omega * invD[3 * nd + c] * (b[3 * nd + c] - kv);
+            const T uo = BCSYM ? pin[c] : uc;
+            o[c] = uo + omega * dd[c] * (bb[c] - kv);
           } else {
-            o = b[3 * nd + c] - kv;
+            o[c] = bb[c] - kv;
           }
-          out[3 * nd + c] = o;
         }
+        T *po = out + d0;
+        po[0] = o[0];
+        po[1] = o[1];
+        po[2] = o[2];
       }
     }
 #pragma unroll
-    for (int py = 0; py < 2; py++)
+    for (int f = 0; f < 4; f++)
 #pragma unroll
-      for (int pz = 0; pz < 2; pz++)
+      for (int c = 0; c < 3; c++) pendT[f][c] = g[2 * f][c] - g[2 * f + 1][c];
+  };
+
+  T Fa[4][3], Fb[4][3], ua[3], ub[3];
+  load_face(i0 - 1, Fa, ua);
+  for (int i = i0 - 1; i <= i1; i++) {
+    step(i, Fa, ua, Fb, ub);
 #pragma unroll
-        for (int c = 0; c < 3; c++) pend[py][pz][c] = y[1][py][pz][c];
+    for (int f = 0; f < 4; f++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) Fa[f][c] = Fb[f][c];
+#pragma unroll
+    for (int c = 0; c < 3; c++) ua[c] = ub[c];
   }
   if (MODE == kModeApply && dotp) {  // one partial per workgroup, fixed order
 #pragma unroll
@@ -543,6 +546,25 @@ __global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_tile_sweep(
       dotp[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = s;
     }
   }
+}
+
+template <typename T, int LVL, int MODE, bool BCSYM>
+__global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_tile_sweep(
+    TileArgs a, std::conditional_t<LVL == 0, Blocks<T>, int> B, const T *__restrict__ E,
+    const T *__restrict__ in, T *__restrict__ out, const T *__restrict__ b,
+    const T *__restrict__ invD, T omega, const uint8_t *__restrict__ fix,
+    double *__restrict__ dotp) {
+  tile_sweep_body<T, LVL, MODE, BCSYM>(a, B, E, in, out, b, invD, omega, fix, dotp);
+}
+
+// fine level in FP32: workgroups of at most 512 threads, which lets the
+// compiler keep the kernel under 72 VGPRs (7 waves per SIMD)
+template <int MODE, bool BCSYM>
+__global__ __launch_bounds__(512) void k_tile_sweep_f32(
+    TileArgs a, Blocks<float> B, const float *__restrict__ E, const float *__restrict__ in,
+    float *__restrict__ out, const float *__restrict__ b, const float *__restrict__ invD,
+    float omega, const uint8_t *__restrict__ fix, double *__restrict__ dotp) {
+  tile_sweep_body<float, 0, MODE, BCSYM>(a, B, E, in, out, b, invD, omega, fix, dotp);
 }
 
 // ---------------------------------------------------------------------------
@@ -614,7 +636,8 @@ struct TileShape {
 static TileShape tile_shape(const Level &L, int lvl, int elem_bytes, int &nx_fixed) {
   TileShape s{};
   const int lds_threads = 60 * 1024 / (18 * elem_bytes);
-  const int max_threads = std::min(elem_bytes == 4 ? 1024 : 512, lds_threads);
+  // launch bounds: 512 threads except level 1 in FP32 (1024)
+  const int max_threads = std::min((elem_bytes == 4 && lvl == 1) ? 1024 : 512, lds_threads);
   struct Rule {
     int ycap, tz, nx;
   };
@@ -698,14 +721,23 @@ static int launch_tile(const DevGrid &g, int mode, const T *E, const T *in, T *o
     return 0;
   }
   const Level &L = g.lv[LVL];
+  if (3 * L.nodes() >= (int64_t(1) << 31) || g.nelem >= (int64_t(1) << 31)) {
+    set_error(TOP3D_ERR_ARG, "grid too large for 32-bit offsets (%lld nodes)",
+              (long long)L.nodes());
+    return 0;
+  }
   int nx_fixed = 0;
   TileShape s = tile_shape(L, LVL, (int)sizeof(T), nx_fixed);
   const int nthr = s.TY * s.TZ;
   const int bs = (nthr + 63) / 64 * 64;
   const size_t shm = sizeof(T) * 2 * 9 * (size_t)nthr;
   static int cached_nthr = -1, cached_slots = 1;  // per instantiation
+  constexpr bool kF32Fine = sizeof(T) == 4 && LVL == 0;
   if (cached_nthr != nthr) {
-    cached_slots = wg_per_cu((const void *)k_tile_sweep<T, LVL, kModeJacobi, false>, bs, shm);
+    const void *kfn;
+    if constexpr (kF32Fine) kfn = (const void *)k_tile_sweep_f32<kModeJacobi, false>;
+    else kfn = (const void *)k_tile_sweep<T, LVL, kModeJacobi, false>;
+    cached_slots = wg_per_cu(kfn, bs, shm);
     cached_nthr = nthr;
   }
   tile_x(s, L, nx_fixed, (int64_t)cached_slots * device_cus());
@@ -731,9 +763,15 @@ static int launch_tile(const DevGrid &g, int mode, const T *E, const T *in, T *o
     B = 0;
     upload_k1();
   }
-#define T3D_LAUNCH(M, S)                                                                    \
-  hipLaunchKernelGGL((k_tile_sweep<T, LVL, M, S>), grid, dim3(bs), shm, stream(), a, B, E, \
-                     in, out, b, invD, omega, g.fixmask[LVL], dotp)
+#define T3D_LAUNCH(M, S)                                                                      \
+  do {                                                                                       \
+    if constexpr (kF32Fine)                                                                  \
+      hipLaunchKernelGGL((k_tile_sweep_f32<M, S>), grid, dim3(bs), shm, stream(), a, B, E,   \
+                         in, out, b, invD, omega, g.fixmask[LVL], dotp);                     \
+    else                                                                                     \
+      hipLaunchKernelGGL((k_tile_sweep<T, LVL, M, S>), grid, dim3(bs), shm, stream(), a, B, \
+                         E, in, out, b, invD, omega, g.fixmask[LVL], dotp);                  \
+  } while (0)
   if (bcsym) {
     if (mode == kModeApply) T3D_LAUNCH(kModeApply, true);
     else if (mode == kModeJacobi) T3D_LAUNCH(kModeJacobi, true);
