@@ -48,6 +48,7 @@ def run_form(top3d_amd, torch, nelx, nely, nelz, nl, precision, warmup, steps):
         print(f"[bench] {precision} warm-up step: solve {r['solve_s']:.4f}s cg {r['cgiter'] + 1}",
               file=sys.stderr, flush=True)
     torch.cuda.synchronize()
+    g.sweep_timing(True)  # HIP events around every fine sweep of the timed solves
     t0 = time.perf_counter()
     for _ in range(steps):
         recs.append(step())
@@ -55,7 +56,22 @@ def run_form(top3d_amd, torch, nelx, nely, nelz, nl, precision, warmup, steps):
               f"cg {recs[-1]['cgiter'] + 1} c {recs[-1]['c']:.6e}", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    sweep_ms, sweeps, sweep_bytes = g.sweep_timing(False)
+    recs[0]["sweep_live"] = {"avg_ms": sweep_ms, "launches": sweeps, "bytes": sweep_bytes}
     return g, st, recs, wall
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE scaled by a reference kernel), or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", f"fine_sweep_f32_{workload}_pmc.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(rf"traffic_bytes_per_launch {kernel}: (\d+)", text)
+    return float(m.group(1)) if m else None
 
 
 def cpu_baseline(nelx, nely, nelz, nl, gpu_iters):
@@ -116,8 +132,8 @@ def main():
         ms, by = g.time_kernel(args.kernel_only, args.kernel_reps)
         src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
+        for _ in range(3):  # one elementwise kernel: 256 MiB read, 256 MiB written
+            torch.add(src, 1.0, out=dst)
         torch.cuda.synchronize()
         print(json.dumps({"kernel": args.kernel_only, "avg_ms": ms, "bytes": by,
                           "copy_bytes_each_way": src.numel() * 4}), flush=True)
@@ -135,9 +151,12 @@ def main():
         ms, by = g.time_kernel(name, args.kernel_reps)
         kernels[name] = {"avg_ms": ms, "bytes": by, "GB_s": by / (ms * 1e-3) / 1e9,
                          "hbm_frac": by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
-    # dominant kernel of the mixed solve: the larger per-CG-iteration time
-    dom = max(("fine_sweep_f32", "level1_sweep_f32"),
-              key=lambda n: kernels[n]["avg_ms"] * SWEEPS_PER_VCYCLE)
+    # dominant kernel of the mixed solve: the FP32 fine sweep (7 per V-cycle),
+    # timed live over the timed design iterations
+    live = recs[0]["sweep_live"]
+    dom = "fine_sweep_f32"
+    traffic = measured_traffic(dom, f"{nelx}x{nely}x{nelz}")
+    live_gbs = live["bytes"] / (live["avg_ms"] * 1e-3) / 1e9 if live["avg_ms"] > 0 else 0.0
     g.close()
 
     fp64 = None
@@ -151,6 +170,7 @@ def main():
                 "cg_iters": [r["cgiter"] + 1 for r in recs64],
                 "compliance": [r["c"] for r in recs64],
                 "mixed_vs_fp64_max_rel_compliance_diff": cdiff,
+                "fine_sweep_f64_live": recs64[0]["sweep_live"],
                 "speedup_mixed_over_fp64": s64 / solve_s}
         g64.close()
 
@@ -161,7 +181,6 @@ def main():
         print("[bench] CPU baseline (oracle) ...", file=sys.stderr, flush=True)
         cpu = cpu_baseline(nelx, nely, nelz, nl, iters)
 
-    k = kernels[dom]
     out = {
         "metric": "MGCG solve s/design-iter at 256x128x128, mixed vs FP64 @rtol 1e-5; matvec GB/s",
         "value": solve_s,
@@ -180,9 +199,10 @@ def main():
                                f"{nl} levels, V(4,4), rtol 1e-5",
                    "elements": nelx * nely * nelz, "dofs": dofs, "levels": nl,
                    "precision": "mixed"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": k["GB_s"],
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": k["GB_s"] / HBM_PEAK_GBS,
-                     "traffic": None},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": live_gbs,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": live_gbs / HBM_PEAK_GBS,
+                     "traffic": traffic, "algorithmic_bytes": live["bytes"],
+                     "avg_launch_ms": live["avg_ms"], "launches_timed": live["launches"]},
         "cpu_baseline": cpu,
         "cg_iters": cg_mixed,
         "compliance": [r["c"] for r in recs],

@@ -317,10 +317,19 @@ int top3d_compliance_backward(const struct gpuGrid *gpu_gc, double rmin,
 int top3d_time_kernel(struct SolverDataBuffer *data, int which, int reps,
                       double *avg_ms, double *bytes);
 
+/* Live timing of the fine-level damped-Jacobi sweeps inside the V-cycles of
+ * the following solves: HIP events around each sweep launch on the library
+ * stream.  enable = 1 clears the accumulators and starts recording, 0 stops.
+ * Outputs (any may be NULL): mean milliseconds per sweep so far, number of
+ * sweeps timed, algorithmic bytes of one sweep (as top3d_time_kernel 0 / 1
+ * for the mixed / FP64 form).                                               */
+int top3d_sweep_timing(struct SolverDataBuffer *data, int enable, double *avg_ms,
+                       int *count, double *bytes);
+
 /* The coarsest-level dense solver on its own: A (device, n x n row-major,
  * symmetric positive definite, nonzero only within |i-j| <= bw) is replaced
- * by A^{-1} (banded Cholesky on FP64 MFMA, then banded solves).  Returns
- * TOP3D_OK or TOP3D_ERR_NOT_SPD.                                            */
+ * by A^{-1} (banded Cholesky on FP64 MFMA, W = L^{-1}, A^{-1} = W^T W).
+ * Returns TOP3D_OK or TOP3D_ERR_NOT_SPD.                                    */
 int top3d_dense_spd_inverse(double *A_dev, int n, int bw);
 
 /* FP32 variants of the level operators, the arithmetic of the mixed

@@ -103,6 +103,8 @@ void SolverState::release() {
   f(scal); f(partials); f(cgpart);
   if (hscal) hipHostFree(hscal);
   if (ev) hipEventDestroy(ev);
+  for (hipEvent_t e : sw_ev) hipEventDestroy(e);
+  sw_ev.clear();
   magic = 0;
 }
 
@@ -137,7 +139,20 @@ template <typename T>
 void SolverState::op(int l, int mode, const T *in, T *out, const T *b) {
   const T omega = (T)kOmega;
   const T *invD = invd<T>(l);
-  if (l == 0)
+  if (l == 0 && mode == 1 && sweep_timing) {
+    while (sw_ev.size() < sw_used + 2) {
+      hipEvent_t e = nullptr;
+      if (!T3D_HIP(hipEventCreate(&e))) break;
+      sw_ev.push_back(e);
+    }
+    const bool rec = sw_ev.size() >= sw_used + 2;
+    if (rec) hipEventRecord(sw_ev[sw_used], stream());
+    launch_fine<T>(*g, mode, young_t<T>(), in, out, b, invD, omega, bcsym);
+    if (rec) {
+      hipEventRecord(sw_ev[sw_used + 1], stream());
+      sw_used += 2;
+    }
+  } else if (l == 0)
     launch_fine<T>(*g, mode, young_t<T>(), in, out, b, invD, omega, bcsym);
   else if (l == 1)
     launch_level1<T>(*g, mode, young_t<T>(), in, out, b, invD, omega, bcsym);
@@ -197,6 +212,17 @@ void SolverState::precondition() {
     return;
   }
   vcycle<float>(0);
+}
+
+void SolverState::collect_sweep_timing() {
+  for (size_t k = 0; k + 1 < sw_used; k += 2) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, sw_ev[k], sw_ev[k + 1]) == hipSuccess) {
+      sw_ms += ms;
+      sw_count++;
+    }
+  }
+  sw_used = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -278,6 +304,7 @@ int SolverState::solve(const double *x, const double *b, double *u, double tol, 
   stats.relres = relres;
   stats.status = status;
   stats.solve_ms = std::chrono::duration<double, std::milli>(clk::now() - t1).count();
+  if (sweep_timing) collect_sweep_timing();
   return status;
 }
 
@@ -355,6 +382,26 @@ void top3d_set_max_iter(struct SolverDataBuffer *data, int max_iter) {
 void top3d_solver_stats(const struct SolverDataBuffer *data, struct Top3dSolveStats *out) {
   SolverState *s = solver_state(data);
   if (s && out) *out = s->stats;
+}
+
+int top3d_sweep_timing(struct SolverDataBuffer *data, int enable, double *avg_ms, int *count,
+                       double *bytes) {
+  SolverState *s = solver_state(data);
+  if (!s) return top3d_last_error();
+  if (enable) {
+    s->sw_ms = 0.0;
+    s->sw_count = 0;
+    s->sw_used = 0;
+  }
+  s->sweep_timing = enable != 0;
+  if (avg_ms) *avg_ms = s->sw_count ? s->sw_ms / s->sw_count : 0.0;
+  if (count) *count = s->sw_count;
+  if (bytes) {
+    const DevGrid &g = *s->g;
+    const double nn0 = (double)g.lv[0].interior_nodes(), ne0 = (double)g.lv[0].elements();
+    *bytes = s->mixed ? 48.0 * nn0 + 4.0 * ne0 : 96.0 * nn0 + 8.0 * ne0;
+  }
+  return TOP3D_OK;
 }
 
 int top3d_time_kernel(struct SolverDataBuffer *data, int which, int reps, double *avg_ms,

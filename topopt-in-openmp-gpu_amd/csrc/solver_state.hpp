@@ -40,6 +40,13 @@ struct SolverState {
   double *cgpart = nullptr;  // partial sums of r.z, p.q, r.r
   hipEvent_t ev = nullptr;
   Top3dSolveStats stats{};
+  // live timing of the fine sweeps (top3d_sweep_timing)
+  bool sweep_timing = false;
+  std::vector<hipEvent_t> sw_ev;
+  size_t sw_used = 0;
+  double sw_ms = 0.0;
+  int sw_count = 0;
+  void collect_sweep_timing();
 
   bool allocate(DevGrid *grid, int nlev);
   void release();

@@ -147,6 +147,7 @@ SIGNATURES = {
     "top3d_compliance_backward": (c_int, [P, c_double, P, P, c_double, P]),
     "top3d_dense_spd_inverse": (c_int, [P, c_int, c_int]),
     "top3d_time_kernel": (c_int, [P, c_int, c_int, P, P]),
+    "top3d_sweep_timing": (c_int, [P, c_int, P, P, P]),
     "applyStateOperator_stencil_f32": (None, [P, P, P, P]),
     "applyStateOperatorSubspace_halo_f32": (None, [P, c_int, P, P, P]),
 }
@@ -413,6 +414,17 @@ class Grid:
         if rc:
             check("top3d_time_kernel")
         return ms.value, by.value
+
+
+    def sweep_timing(self, enable):
+        """Start (enable=True) or stop live timing of the fine sweeps inside the
+        V-cycles; returns (mean ms per sweep, sweeps timed, bytes per sweep)."""
+        ms, cnt, by = c_double(), c_int(), c_double()
+        rc = self.lib.top3d_sweep_timing(ctypes.byref(self.data), int(bool(enable)),
+                                         ctypes.byref(ms), ctypes.byref(cnt), ctypes.byref(by))
+        if rc:
+            check("top3d_sweep_timing")
+        return ms.value, cnt.value, by.value
 
 
 class DesignState:
