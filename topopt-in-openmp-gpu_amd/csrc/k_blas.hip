@@ -134,6 +134,7 @@ __global__ __launch_bounds__(kRedBlock) void k_dot_rz(const double *__restrict__
                                                       const TZ *__restrict__ z, int64_t n,
                                                       double *__restrict__ partials) {
   double s = 0;
+#pragma unroll 4
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     s += r[i] * (double)z[i];
@@ -150,6 +151,7 @@ __global__ __launch_bounds__(kRedBlock) void k_pupdate_fused(
   const double rho = sum_partials_all(partials, npart);
   if (blockIdx.x == 0 && threadIdx.x == 0) s[0] = rho;
   const double beta = first ? 0.0 : rho / s[1];
+#pragma unroll 4
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const double zi = (double)z[i];

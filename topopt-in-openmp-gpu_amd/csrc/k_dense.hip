@@ -566,7 +566,7 @@ int CoarseFactor::factor() {
 // ---------------------------------------------------------------------------
 // Coarsest solve z = W^T (W r): gather the interior DOFs, lower-triangular
 // GEMV, upper-triangular GEMV scattered back to the halo layout (the halo of
-// z stays zero).  One wavefront per row, FP64 accumulation.
+// z stays zero).  One workgroup per row, FP64 accumulation.
 template <typename T>
 __global__ void k_gather_interior(int nx, int ny, int nz, int wy, int wz,
                                   const T *__restrict__ in, T *__restrict__ out) {
@@ -578,18 +578,26 @@ __global__ void k_gather_interior(int nx, int ny, int nz, int wy, int wz,
   out[d] = in[3 * (((int64_t)(i + 1) * wz + (k + 1)) * wy + (j + 1)) + d % 3];
 }
 
+// One 256-thread workgroup per row: the row (up to n values) is read by all
+// four waves at once, FP64 partial sums, fixed-order block reduction.
+__device__ __forceinline__ double row_block_sum(double s) {
+  __shared__ double sm[4];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = s;
+  __syncthreads();
+  return (sm[0] + sm[1]) + (sm[2] + sm[3]);  // meaningful in thread 0
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_trmv_lower(const T *__restrict__ Wm, int n,
                                                     const T *__restrict__ x, T *__restrict__ y) {
-  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (row >= n) return;
+  const int row = blockIdx.x;
   const T *a = Wm + (int64_t)row * n;
   double s = 0.0;
-  for (int c = lane; c <= row; c += 64) s += (double)a[c] * (double)x[c];
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-  if (lane == 0) y[row] = (T)s;
+  for (int c = threadIdx.x; c <= row; c += 256) s += (double)a[c] * (double)x[c];
+  s = row_block_sum(s);
+  if (threadIdx.x == 0) y[row] = (T)s;
 }
 
 template <typename T>
@@ -597,15 +605,12 @@ __global__ __launch_bounds__(256) void k_trmv_upper_scatter(int nx, int ny, int 
                                                             const T *__restrict__ WTm, int n,
                                                             const T *__restrict__ y,
                                                             T *__restrict__ out) {
-  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (row >= n) return;
+  const int row = blockIdx.x;
   const T *a = WTm + (int64_t)row * n;
   double s = 0.0;
-  for (int c = row + lane; c < n; c += 64) s += (double)a[c] * (double)y[c];
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-  if (lane == 0) {
+  for (int c = row + threadIdx.x; c < n; c += 256) s += (double)a[c] * (double)y[c];
+  s = row_block_sum(s);
+  if (threadIdx.x == 0) {
     const int64_t p = row / 3;
     const int j = (int)(p % ny), k = (int)((p / ny) % nz), i = (int)(p / ((int64_t)ny * nz));
     out[3 * (((int64_t)(i + 1) * wz + (k + 1)) * wy + (j + 1)) + row % 3] = (T)s;
@@ -621,8 +626,8 @@ void launch_coarse_apply(const DevGrid &g, int l, const CoarseFactor &cf, const 
   const int bs = 256;
   hipLaunchKernelGGL((k_gather_interior<T>), dim3((n + bs - 1) / bs), dim3(bs), 0, stream(),
                      L.nx, L.ny, L.nz, L.wy, L.wz, in, x);
-  hipLaunchKernelGGL((k_trmv_lower<T>), dim3((n + 3) / 4), dim3(256), 0, stream(), Wm, n, x, y);
-  hipLaunchKernelGGL((k_trmv_upper_scatter<T>), dim3((n + 3) / 4), dim3(256), 0, stream(), L.nx,
+  hipLaunchKernelGGL((k_trmv_lower<T>), dim3(n), dim3(256), 0, stream(), Wm, n, x, y);
+  hipLaunchKernelGGL((k_trmv_upper_scatter<T>), dim3(n), dim3(256), 0, stream(), L.nx,
                      L.ny, L.nz, L.wy, L.wz, WTm, n, y, out);
 }
 
