@@ -358,17 +358,15 @@ void launch_stencil(const DevGrid &g, int l, int mode, const T *S, const T *in,
 
 // First pre-smoothing sweep from z = 0 (K 0 = 0): out = w invD b inside.
 template <typename TB, typename T>
-__global__ void k_jacobi_zero(int64_t ndof, int nx, int ny, int nz, int wy, int wz,
-                              const TB *__restrict__ b, const T *__restrict__ invD,
-                              T omega, T *__restrict__ out) {
-  const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (d >= ndof) return;
-  const int64_t node = d / 3;
-  const int j = (int)(node % wy);
-  const int k = (int)((node / wy) % wz);
-  const int i = (int)(node / ((int64_t)wy * wz));
-  const bool inside = i >= 1 && i <= nx && j >= 1 && j <= ny && k >= 1 && k <= nz;
-  out[d] = inside ? (T)(omega * invD[d] * ((T)b[d] - T(0))) : T(0);
+__global__ __launch_bounds__(256) void k_jacobi_zero(int64_t ndof, const TB *__restrict__ b,
+                                                   const T *__restrict__ invD, T omega,
+                                                   T *__restrict__ out) {
+  // invD is zero on every non-real node of a level (k_invdiag_*), so the
+  // sweep from zero needs no node classification
+#pragma unroll 4
+  for (int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; d < ndof;
+       d += (int64_t)gridDim.x * blockDim.x)
+    out[d] = (T)(omega * invD[d] * ((T)b[d] - T(0)));
 }
 
 template <typename TB, typename T>
@@ -376,8 +374,9 @@ void launch_jacobi_zero(const DevGrid &g, int l, const TB *b, const T *invD, T o
                         T *out) {
   const Level &L = g.lv[l];
   const int bs = 256;
-  hipLaunchKernelGGL((k_jacobi_zero<TB, T>), dim3((L.ndof + bs - 1) / bs), dim3(bs), 0,
-                     stream(), L.ndof, L.nx, L.ny, L.nz, L.wy, L.wz, b, invD, omega, out);
+  const int64_t blocks = std::min<int64_t>((L.ndof + bs - 1) / bs, 4096);
+  hipLaunchKernelGGL((k_jacobi_zero<TB, T>), dim3(blocks), dim3(bs), 0, stream(), L.ndof, b,
+                     invD, omega, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -423,45 +422,66 @@ void launch_restrict(const DevGrid &g, int l, const T *fine, T *coarse) {
 // Trilinear prolongation level l+1 -> l (stencil_methods.c:521-590);
 // ADD = true fuses the correction z += P zc of the V-cycle.
 template <typename T, bool ADD>
-__global__ void k_prolong(int fnx, int fny, int fnz, int fwy, int fwz, int cwy, int cwz,
-                          const T *__restrict__ coarse, T *__restrict__ fine) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int k = blockIdx.y * blockDim.y + threadIdx.y;
-  const int i = blockIdx.z;
-  if (j >= fwy || k >= fwz) return;
-  const int64_t fn = ((int64_t)i * fwz + k) * fwy + j;
-  // only interior fine nodes are written (stencil_methods.c:546-589)
-  if (!(i >= 1 && i <= fnx && j >= 1 && j <= fny && k >= 1 && k <= fnz)) return;
-  const int i1 = (i - 1) / 2 + 1, i2 = i / 2 + 1;
-  const int j1 = (j - 1) / 2 + 1, j2 = j / 2 + 1;
-  const int k1 = (k - 1) / 2 + 1, k2 = k / 2 + 1;
-  const int64_t sx = (int64_t)cwy * cwz;
-  const int64_t c[8] = {i1 * sx + k1 * cwy + j2, i2 * sx + k1 * cwy + j2,
-                        i2 * sx + k1 * cwy + j1, i1 * sx + k1 * cwy + j1,
-                        i1 * sx + k2 * cwy + j2, i2 * sx + k2 * cwy + j2,
-                        i2 * sx + k2 * cwy + j1, i1 * sx + k2 * cwy + j1};
+__global__ void k_prolong(int cnx, int cny, int cnz, int cwy, int cwz, int fnx, int fny, int fnz,
+                          int fwy, int fwz, const T *__restrict__ coarse, T *__restrict__ fine) {
+  // thread per coarse node (I, J, K): the fine nodes 2I-1, 2I (and the same
+  // in y, z) take their trilinear weights from the 8 coarse nodes I..I+1,
+  // J..J+1, K..K+1, loaded once; per fine node the sum runs over the same 8
+  // terms in the same order as stencil_methods.c:546-589
+  const int J = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int K = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  const int I = blockIdx.z + 1;
+  if (J > cny || K > cnz || I > cnx) return;
+  const int64_t csx = (int64_t)cwy * cwz;
+  T cv[2][2][2][3];  // [di][dk][dj][comp]
 #pragma unroll
-  for (int d = 0; d < 3; d++) {
-    T s = 0;
+  for (int di = 0; di < 2; di++)
 #pragma unroll
-    for (int q = 0; q < 8; q++) s += T(0.125) * coarse[3 * c[q] + d];
-    if (ADD) fine[3 * fn + d] += s; else fine[3 * fn + d] = s;
-  }
+    for (int dk = 0; dk < 2; dk++)
+#pragma unroll
+      for (int dj = 0; dj < 2; dj++) {
+        const T *pc = coarse + 3 * ((I + di) * csx + (int64_t)(K + dk) * cwy + (J + dj));
+#pragma unroll
+        for (int c = 0; c < 3; c++) cv[di][dk][dj][c] = pc[c];
+      }
+  const int64_t fsx = (int64_t)fwy * fwz;
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int cc = 0; cc < 2; cc++)
+#pragma unroll
+      for (int bb = 0; bb < 2; bb++) {
+        const int i = 2 * I - 1 + a, k = 2 * K - 1 + cc, j = 2 * J - 1 + bb;
+        if (i > fnx || j > fny || k > fnz) continue;
+        // i1 = (i-1)/2+1, i2 = i/2+1 relative to I: offsets 0 or a
+        const int i1 = 0, i2 = a, j1 = 0, j2 = bb, k1 = 0, k2 = cc;
+        const int q[8][3] = {{i1, k1, j2}, {i2, k1, j2}, {i2, k1, j1}, {i1, k1, j1},
+                             {i1, k2, j2}, {i2, k2, j2}, {i2, k2, j1}, {i1, k2, j1}};
+        T *pf = fine + 3 * (i * fsx + (int64_t)k * fwy + j);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          T s = 0;
+#pragma unroll
+          for (int t = 0; t < 8; t++) s += T(0.125) * cv[q[t][0]][q[t][1]][q[t][2]][d];
+          if (ADD) pf[d] += s; else pf[d] = s;
+        }
+      }
 }
 
+template <typename T, bool ADD>
+static void launch_prolong_t(const DevGrid &g, int l, const T *coarse, T *fine) {
+  const Level &C = g.lv[l + 1], &F = g.lv[l];
+  dim3 bs(64, 4, 1), gs((C.ny + 63) / 64, (C.nz + 3) / 4, C.nx);
+  hipLaunchKernelGGL((k_prolong<T, ADD>), gs, bs, 0, stream(), C.nx, C.ny, C.nz, C.wy, C.wz,
+                     F.nx, F.ny, F.nz, F.wy, F.wz, coarse, fine);
+}
 template <typename T>
 void launch_prolong_add(const DevGrid &g, int l, const T *coarse, T *fine) {
-  const Level &C = g.lv[l + 1], &F = g.lv[l];
-  dim3 bs(64, 4, 1), gs((F.wy + 63) / 64, (F.wz + 3) / 4, F.wx);
-  hipLaunchKernelGGL((k_prolong<T, true>), gs, bs, 0, stream(), F.nx, F.ny, F.nz, F.wy,
-                     F.wz, C.wy, C.wz, coarse, fine);
+  launch_prolong_t<T, true>(g, l, coarse, fine);
 }
 template <typename T>
 void launch_prolong(const DevGrid &g, int l, const T *coarse, T *fine) {
-  const Level &C = g.lv[l + 1], &F = g.lv[l];
-  dim3 bs(64, 4, 1), gs((F.wy + 63) / 64, (F.wz + 3) / 4, F.wx);
-  hipLaunchKernelGGL((k_prolong<T, false>), gs, bs, 0, stream(), F.nx, F.ny, F.nz, F.wy,
-                     F.wz, C.wy, C.wz, coarse, fine);
+  launch_prolong_t<T, false>(g, l, coarse, fine);
 }
 
 // ---------------------------------------------------------------------------
