@@ -624,10 +624,10 @@ struct TileShape {
 
 // Tile of TY x TZ element columns marching over NX node planes.  LDS holds
 // two exchange buffers of 9 values per thread (under 60 KiB).  Tuning
-// override: TOP3D_TILE_L<level>=TY,TZ,NX.
+// override: TOP3D_TILE_L<level>=TY,TZ,NX (NX = 0: from the occupancy).
 //
 // Shapes measured on MI355X at 256x128x128 (fine) / 128x64x64 (level 1):
-// small workgroups (200-300 threads) for the fine level, so that several
+// small workgroups (about 200-260 threads) for the fine level, so that several
 // share a CU and their barriers interleave; wide, short x-marches for level
 // 1.  The x-march length follows from the number of workgroups the device
 // holds at once (`slots`): a launch that overflows them by a few
@@ -641,7 +641,7 @@ static TileShape tile_shape(const Level &L, int lvl, int elem_bytes, int &nx_fix
   struct Rule {
     int ycap, tz, nx;
   };
-  static constexpr Rule kRules[2][2] = {{{33, 6, 0}, {33, 8, 0}},   // level 0: f64, f32
+  static constexpr Rule kRules[2][2] = {{{17, 12, 0}, {33, 8, 0}},  // level 0: f64, f32
                                         {{65, 6, 8}, {65, 12, 4}}};  // level 1
   const Rule r = kRules[lvl == 0 ? 0 : 1][elem_bytes == 4 ? 1 : 0];
   s.TY = std::max(2, std::min(L.nely + 1, r.ycap));
@@ -649,7 +649,7 @@ static TileShape tile_shape(const Level &L, int lvl, int elem_bytes, int &nx_fix
   nx_fixed = r.nx;
   const char *env = std::getenv(lvl == 0 ? "TOP3D_TILE_L0" : "TOP3D_TILE_L1");
   int oy = 0, oz = 0, ox = 0;
-  if (env && std::sscanf(env, "%d,%d,%d", &oy, &oz, &ox) == 3 && oy >= 2 && oz >= 2 && ox >= 1 &&
+  if (env && std::sscanf(env, "%d,%d,%d", &oy, &oz, &ox) == 3 && oy >= 2 && oz >= 2 && ox >= 0 &&
       oy * oz <= max_threads) {
     s.TY = std::min(oy, std::max(2, L.nely + 1));
     s.TZ = std::min(oz, std::max(2, L.nelz + 1));

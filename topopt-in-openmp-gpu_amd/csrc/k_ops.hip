@@ -385,13 +385,13 @@ void launch_jacobi_zero(const DevGrid &g, int l, const TB *b, const T *invD, T o
 template <typename T>
 __global__ void k_restrict(int cnx, int cny, int cnz, int cwy, int cwz, int fwy, int fwz,
                            const T *__restrict__ fine, T *__restrict__ coarse) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int k = blockIdx.y * blockDim.y + threadIdx.y;
-  const int i = blockIdx.z;
-  if (j >= cwy || k >= cwz) return;
+  // one thread per interior coarse node; only those are written
+  // (stencil_methods.c:629-686)
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)cnx * cny * cnz) return;
+  const int j = (int)(t % cny) + 1, k = (int)((t / cny) % cnz) + 1,
+            i = (int)(t / ((int64_t)cny * cnz)) + 1;
   const int64_t cn = ((int64_t)i * cwz + k) * cwy + j;
-  // only interior coarse nodes are written (stencil_methods.c:629-686)
-  if (!(i >= 1 && i <= cnx && j >= 1 && j <= cny && k >= 1 && k <= cnz)) return;
   const int fi = 2 * i - 1, fj = 2 * j - 1, fk = 2 * k - 1;
   T acc[3] = {0, 0, 0};
 #pragma unroll
@@ -414,9 +414,9 @@ __global__ void k_restrict(int cnx, int cny, int cnz, int cwy, int cwz, int fwy,
 template <typename T>
 void launch_restrict(const DevGrid &g, int l, const T *fine, T *coarse) {
   const Level &C = g.lv[l + 1], &F = g.lv[l];
-  dim3 bs(64, 4, 1), gs((C.wy + 63) / 64, (C.wz + 3) / 4, C.wx);
-  hipLaunchKernelGGL((k_restrict<T>), gs, bs, 0, stream(), C.nx, C.ny, C.nz, C.wy, C.wz,
-                     F.wy, F.wz, fine, coarse);
+  const int64_t nth = C.interior_nodes();
+  hipLaunchKernelGGL((k_restrict<T>), dim3((nth + 255) / 256), dim3(256), 0, stream(), C.nx,
+                     C.ny, C.nz, C.wy, C.wz, F.wy, F.wz, fine, coarse);
 }
 
 // Trilinear prolongation level l+1 -> l (stencil_methods.c:521-590);
@@ -428,10 +428,10 @@ __global__ void k_prolong(int cnx, int cny, int cnz, int cwy, int cwz, int fnx, 
   // in y, z) take their trilinear weights from the 8 coarse nodes I..I+1,
   // J..J+1, K..K+1, loaded once; per fine node the sum runs over the same 8
   // terms in the same order as stencil_methods.c:546-589
-  const int J = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  const int K = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  const int I = blockIdx.z + 1;
-  if (J > cny || K > cnz || I > cnx) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)cnx * cny * cnz) return;
+  const int J = (int)(t % cny) + 1, K = (int)((t / cny) % cnz) + 1,
+            I = (int)(t / ((int64_t)cny * cnz)) + 1;
   const int64_t csx = (int64_t)cwy * cwz;
   T cv[2][2][2][3];  // [di][dk][dj][comp]
 #pragma unroll
@@ -471,9 +471,9 @@ __global__ void k_prolong(int cnx, int cny, int cnz, int cwy, int cwz, int fnx, 
 template <typename T, bool ADD>
 static void launch_prolong_t(const DevGrid &g, int l, const T *coarse, T *fine) {
   const Level &C = g.lv[l + 1], &F = g.lv[l];
-  dim3 bs(64, 4, 1), gs((C.ny + 63) / 64, (C.nz + 3) / 4, C.nx);
-  hipLaunchKernelGGL((k_prolong<T, ADD>), gs, bs, 0, stream(), C.nx, C.ny, C.nz, C.wy, C.wz,
-                     F.nx, F.ny, F.nz, F.wy, F.wz, coarse, fine);
+  const int64_t nth = C.interior_nodes();
+  hipLaunchKernelGGL((k_prolong<T, ADD>), dim3((nth + 255) / 256), dim3(256), 0, stream(), C.nx,
+                     C.ny, C.nz, C.wy, C.wz, F.nx, F.ny, F.nz, F.wy, F.wz, coarse, fine);
 }
 template <typename T>
 void launch_prolong_add(const DevGrid &g, int l, const T *coarse, T *fine) {
