@@ -533,8 +533,11 @@ void initializeCholmod(const struct gridContext gc, const int l,
   if (!g) return;
   CoarseObj *c = new CoarseObj();
   c->l = l;
-  const int bw = 3 * (g->lv[l].ny * g->lv[l].nz + g->lv[l].ny + 1) + 2;
-  if (!c->f.allocate(M.nrows, bw, false)) {
+  const Level &L = g->lv[l];
+  const int bw = 3 * (L.ny * L.nz + L.ny + 1) + 2;
+  // the rows are the level's interior DOFs, x-planes of 3 ny nz consecutive
+  const int planes = M.nrows == 3 * L.interior_nodes() ? L.nx : 0;
+  if (!c->f.allocate(M.nrows, bw, false, planes)) {
     c->f.release();
     delete c;
     return;
@@ -554,16 +557,19 @@ void finishCholmod(const struct gridContext gc, const int l, struct CoarseSolver
   (void)gc; (void)l; (void)M;
 }
 
+// CSR rows into the coarse factor's storage (logical DOF d at pos[d])
 __global__ void k_csr_lower_to_dense(int64_t nrows, const int *__restrict__ ro,
                                      const int *__restrict__ ci, const double *__restrict__ va,
+                                     const int *__restrict__ pos, int64_t np,
                                      double *__restrict__ A) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrows) return;
   for (int e = ro[r]; e < ro[r + 1]; e++) {
     const int64_t c = ci[e];
     if (c <= r) {  // CHOLMOD stype=1 on the CSR-as-CSC reads these entries
-      A[r * nrows + c] = va[e];
-      A[c * nrows + r] = va[e];
+      const int64_t pr = pos[r], pc = pos[c];
+      A[pr * np + pc] = va[e];
+      A[pc * np + pr] = va[e];
     }
   }
 }
@@ -579,10 +585,10 @@ void factorizeSubspaceMatrix(const struct gridContext gc, const int l,
   int *ro, *ci;
   double *va;
   upload_csr(M, &ro, &ci, &va);
-  launch_fill<double>((int64_t)c->f.n * c->f.n, c->f.A, 0.0);
+  c->f.begin_assembly();
   const int bs = 256;
   hipLaunchKernelGGL(k_csr_lower_to_dense, dim3((c->f.n + bs - 1) / bs), dim3(bs), 0, stream(),
-                     (int64_t)c->f.n, ro, ci, va, c->f.A);
+                     (int64_t)c->f.n, ro, ci, va, c->f.pos, (int64_t)c->f.np, c->f.A);
   c->f.factor();
   dfree(ro);
   dfree(ci);

@@ -196,7 +196,8 @@ void launch_dot_rz(const double *r, const TZ *z, int64_t n, double *partials) {
 template <typename TZ>
 void launch_pupdate_fused(int64_t n, const double *partials, double *scal, double *p,
                           const TZ *z, int first) {
-  hipLaunchKernelGGL((k_pupdate_fused<TZ>), dim3(kRedGrid), dim3(kRedBlock), 0, stream(), n,
+  // 2048 blocks: 8 waves per SIMD on 256 CUs for this streaming update
+  hipLaunchKernelGGL((k_pupdate_fused<TZ>), dim3(2 * kRedGrid), dim3(kRedBlock), 0, stream(), n,
                      partials, kRedGrid, scal, p, z, first);
 }
 void launch_axpy_fused(int64_t n, const double *pq_part, int npart, double *scal, double *u,

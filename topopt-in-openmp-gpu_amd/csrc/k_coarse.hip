@@ -216,9 +216,11 @@ void launch_node_stencil(const DevGrid &g, int l, const double *Ke, double *S,
                      L.nelx, L.nely, L.nelz, L.wy, L.wz, Ke, g.fixmask[l], S, bc_symmetric);
 }
 
-// dense n x n matrix (row-major) of the assembled coarsest level
+// the assembled coarsest level into the coarse factor's np x np storage
+// (row-major; logical DOF d at storage index pos[d])
 __global__ void k_stencil_to_dense(int nx, int ny, int nz, const double *__restrict__ S,
-                                   double *__restrict__ A, int64_t n) {
+                                   const int *__restrict__ pos, double *__restrict__ A,
+                                   int64_t np) {
   const int64_t nint = (int64_t)nx * ny * nz;
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= 27 * nint) return;
@@ -230,16 +232,17 @@ __global__ void k_stencil_to_dense(int nx, int ny, int nz, const double *__restr
   const int64_t q = ((int64_t)i2 * nz + k2) * ny + j2;
   for (int r = 0; r < 3; r++)
     for (int c = 0; c < 3; c++)
-      A[(3 * p + r) * n + 3 * q + c] = S[(int64_t)(nb * 9 + 3 * r + c) * nint + p];
+      A[(int64_t)pos[3 * p + r] * np + pos[3 * q + c]] =
+          S[(int64_t)(nb * 9 + 3 * r + c) * nint + p];
 }
 
-void launch_stencil_to_dense(const DevGrid &g, int l, const double *S, double *A, int n) {
+void launch_stencil_to_dense(const DevGrid &g, int l, const double *S, CoarseFactor &cf) {
   const Level &L = g.lv[l];
-  launch_fill<double>((int64_t)n * n, A, 0.0);
+  cf.begin_assembly();
   const int64_t nthreads = 27 * L.interior_nodes();
   const int bs = 256;
   hipLaunchKernelGGL(k_stencil_to_dense, dim3((nthreads + bs - 1) / bs), dim3(bs), 0,
-                     stream(), L.nx, L.ny, L.nz, S, A, (int64_t)n);
+                     stream(), L.nx, L.ny, L.nz, S, cf.pos, cf.A, (int64_t)cf.np);
 }
 
 }  // namespace t3d

@@ -2,27 +2,29 @@
 // stencil_assembly.c:366-503).
 //
 // The coarsest matrix (n = 3 * interior nodes, 4131 at 16x8x8 elements) is
-// banded: with the x-slowest node order its half bandwidth is
-// 3*(ny*nz + ny + 1) + 2, and Cholesky keeps that band.  Per design iteration:
-//   1. banded right-looking Cholesky A = L L^T over 64x64 tiles, one launch
-//      per block column b: each workgroup owns one tile (I, J) of the trailing
-//      band, forms the panel tiles Q_I = A_Ib L_bb^{-T} and Q_J on FP64 MFMA
-//      (v_mfma_f64_16x16x4f64) and applies C_IJ -= Q_I Q_J^T; the diagonal
-//      workgroups write the panel of L, and the workgroup of tile
-//      (b+1, b+1) factors it in LDS right after its update (Cholesky, then
-//      the inverse of the 64x64 factor);
-//   2. W = L^{-1} (dense lower triangle), one workgroup per 16-column strip
-//      marching down the block rows with the last band of block rows in LDS,
-//      every step a band GEMM plus a product with the inverse diagonal block,
-//      both on MFMA; W^T is written alongside.
-// A V-cycle applies A^{-1} = W^T W as two triangular GEMVs.
+// block tridiagonal over the grid's x-planes (3 ny nz DOFs each, x slowest).
+// Per design iteration:
+//   1. Cholesky P A P^T = L L^T over 64x64 tiles, with the planes in
+//      nested-dissection order (odd-even reduction of the plane chain, each
+//      plane padded to whole tiles): every launch eliminates one tile column
+//      of every plane of the current independent set, so a 17-plane chain
+//      takes 5 levels x 4 columns (+1) launches instead of the 65 of the
+//      banded order.  Each workgroup owns one target tile of the update,
+//      forms the panel tiles Q = A_XK L_KK^{-T} it needs on FP64 MFMA
+//      (v_mfma_f64_16x16x4f64) and applies C_IJ -= sum_K Q_IK Q_JK^T; the
+//      workgroup of a diagonal tile eliminated next factors it in LDS right
+//      after its update (Cholesky, then the inverse of the 64x64 factor);
+//   2. W = L^{-1} (lower triangle, zero outside the reach of each column),
+//      one workgroup per 16-column strip visiting only the tile rows where
+//      its column of W is non-zero; W^T is written alongside.
+// A V-cycle applies A^{-1} = P^T W^T W P as two triangular GEMVs.
 #include <algorithm>
 
 #include "t3d_internal.hpp"
 
 namespace t3d {
 
-constexpr int kNB = 64;         // tile size
+constexpr int kNB = kTile;      // tile size
 constexpr int kLd = kNB + 1;    // LDS row stride of a 64x64 tile
 constexpr int kStrip = 16;      // columns per workgroup of the W = L^{-1} kernel
 
@@ -215,20 +217,27 @@ __device__ __forceinline__ void factor_tile(double *T, double *X, double *Ys, do
 
 // Write the factored diagonal tile (blk, blk): L to Lf, L^{-1} to Dinv.
 __device__ inline void store_diag(const double *T, const double *X, double *__restrict__ Lf,
-                                  double *__restrict__ Dinv, int n, int blk) {
-  const int k0 = blk * kNB;
+                                  double *__restrict__ Dinv, int np, int blk) {
+  const int64_t k0 = (int64_t)blk * kNB;
   for (int e = threadIdx.x; e < kNB * kNB; e += 256) {
     const int r = e >> 6, c = e & 63;
-    if (k0 + r < n && k0 + c < n) Lf[(int64_t)(k0 + r) * n + k0 + c] = T[r * kLd + c];
+    Lf[(k0 + r) * np + k0 + c] = T[r * kLd + c];
     Dinv[(int64_t)blk * kNB * kNB + e] = X[r * kLd + c];
   }
 }
 
-// One block column b of the banded Cholesky (b = -1: factor tile (0, 0)).
-// Workgroup p handles trailing tile (I, J), b+1 <= J <= I <= t1, enumerated
-// row by row.  A keeps the Schur-complement updates; Lf receives L.
-__global__ __launch_bounds__(256) void k_chol_step(double *__restrict__ A, double *__restrict__ Lf,
-                                                   double *__restrict__ Dinv, int n, int b,
+// One task of an elimination step (one workgroup): the trailing tile (I, J),
+// I >= J, receives the updates of the tile columns K[0..nsrc) eliminated in
+// this step,
+//   C_IJ = A_IJ - sum_K Q_IK Q_JK^T,   Q_XK = A_XK L_KK^{-T}
+// (each workgroup forms the two panel tiles it needs on MFMA).  The task of a
+// diagonal tile (I, I) also writes L_IK = Q_IK.  With kTaskFactor the tile is
+// complete after this update and is factored in LDS right away (Cholesky,
+// then the inverse of the 64x64 factor); nsrc = 0 factors A_II as it is.
+// Matrices are np x np row-major, np a multiple of 64.
+__global__ __launch_bounds__(256) void k_chol_task(double *__restrict__ A, double *__restrict__ Lf,
+                                                   double *__restrict__ Dinv, int np,
+                                                   const CholTask *__restrict__ tasks,
                                                    int *__restrict__ info) {
   __shared__ double PI[kNB * kLd];
   __shared__ double PJ[kNB * kLd];
@@ -236,14 +245,16 @@ __global__ __launch_bounds__(256) void k_chol_step(double *__restrict__ A, doubl
   __shared__ double CT[kNB * kLd];
   __shared__ double Ys[3 * 256];
   __shared__ double wbuf[32];
+  const CholTask tk = tasks[blockIdx.x];
   const int tid = threadIdx.x;
-  int fblk = 0;  // diagonal block factored by this workgroup (if any)
-  if (b < 0) {
+  const bool diag = tk.I == tk.J;
+  const int64_t gi = (int64_t)tk.I * kNB, gj = (int64_t)tk.J * kNB;
+  if (tk.nsrc == 0) {
     double rv[16];
 #pragma unroll
     for (int q = 0; q < 16; q++) {
-      const int e = tid + 256 * q, r = e >> 6, c = e & 63;
-      rv[q] = (r < n && c < n) ? A[(int64_t)r * n + c] : (r == c ? 1.0 : 0.0);
+      const int e = tid + 256 * q;
+      rv[q] = A[(gi + (e >> 6)) * np + gi + (e & 63)];
     }
 #pragma unroll
     for (int q = 0; q < 16; q++) {
@@ -251,60 +262,52 @@ __global__ __launch_bounds__(256) void k_chol_step(double *__restrict__ A, doubl
       DV[(e >> 6) * kLd + (e & 63)] = rv[q];
     }
   } else {
-    int p = blockIdx.x, I = b + 1;
-    while (p >= I - b) {
-      p -= I - b;
-      I++;
-    }
-    const int J = b + 1 + p;
-    const int k0 = b * kNB;
-    // stage the panel tiles P_I, P_J, L_bb^{-1} and the target tile C_IJ;
-    // all loads of a thread are issued before its LDS stores
-    {
+    const int w = tid >> 6, lane = tid & 63;
+    const int qr = (w >> 1) * 32, qc = (w & 1) * 32;
+    double4_t cacc[2][2];
+    zero_acc(cacc);
+    for (int s = 0; s < tk.nsrc; s++) {
+      const int64_t gk = (int64_t)tk.K[s] * kNB;
+      // stage A_IK, A_JK, L_KK^{-1} (and C_IJ with the first source); all
+      // loads of a thread are issued before its LDS stores
       double ri[16], rj[16], rd[16], rc[16];
 #pragma unroll
       for (int q = 0; q < 16; q++) {
         const int e = tid + 256 * q, r = e >> 6, c = e & 63;
-        const int64_t gi = (int64_t)I * kNB + r, gj = (int64_t)J * kNB + r,
-                      gjc = (int64_t)J * kNB + c;
-        ri[q] = (gi < n) ? A[gi * n + k0 + c] : 0.0;
-        rj[q] = (J != I && gj < n) ? A[gj * n + k0 + c] : 0.0;
-        rd[q] = Dinv[(int64_t)b * kNB * kNB + e];
-        rc[q] = (gi < n && gjc < n) ? A[gi * n + gjc] : (r == c ? 1.0 : 0.0);
+        ri[q] = A[(gi + r) * np + gk + c];
+        rj[q] = diag ? 0.0 : A[(gj + r) * np + gk + c];
+        rd[q] = Dinv[(int64_t)tk.K[s] * kNB * kNB + e];
+        rc[q] = s == 0 ? A[(gi + r) * np + gj + c] : 0.0;
       }
+      __syncthreads();  // the previous source's products are done with the tiles
 #pragma unroll
       for (int q = 0; q < 16; q++) {
         const int e = tid + 256 * q, o = (e >> 6) * kLd + (e & 63);
         PI[o] = ri[q];
-        if (J != I) PJ[o] = rj[q];
+        if (!diag) PJ[o] = rj[q];
         DV[o] = rd[q];
-        CT[o] = rc[q];
+        if (s == 0) CT[o] = rc[q];
       }
-    }
-    __syncthreads();
-    const int w = tid >> 6, lane = tid & 63;
-    const int qr = (w >> 1) * 32, qc = (w & 1) * 32;
-    double4_t qi[2][2], qj[2][2];
-    zero_acc(qi);
-    zero_acc(qj);
-    mm_nt_tile(PI, DV, qr, qc, lane, qi);  // Q_I = P_I L_bb^{-T}
-    if (J != I) mm_nt_tile(PJ, DV, qr, qc, lane, qj);
-    __syncthreads();
-    store_acc(PI, qi, qr, qc, lane);
-    if (J != I) store_acc(PJ, qj, qr, qc, lane);
-    __syncthreads();
-    if (I == J) {
+      __syncthreads();
+      double4_t qi[2][2], qj[2][2];
+      zero_acc(qi);
+      zero_acc(qj);
+      mm_nt_tile(PI, DV, qr, qc, lane, qi);  // Q_I = A_IK L_KK^{-T}
+      if (!diag) mm_nt_tile(PJ, DV, qr, qc, lane, qj);
+      __syncthreads();
+      store_acc(PI, qi, qr, qc, lane);
+      if (!diag) store_acc(PJ, qj, qr, qc, lane);
+      __syncthreads();
+      if (diag) {
 #pragma unroll 4
-      for (int e = tid; e < kNB * kNB; e += 256) {
-        const int r = e >> 6, c = e & 63;
-        const int64_t gi = (int64_t)I * kNB + r;
-        if (gi < n) Lf[gi * n + k0 + c] = PI[r * kLd + c];
+        for (int e = tid; e < kNB * kNB; e += 256) {
+          const int r = e >> 6, c = e & 63;
+          Lf[(gi + r) * np + gk + c] = PI[r * kLd + c];
+        }
       }
+      mm_nt_tile(PI, diag ? PI : PJ, qr, qc, lane, cacc);  // += Q_I Q_J^T
     }
-    double4_t cacc[2][2];
-    zero_acc(cacc);
-    mm_nt_tile(PI, (J != I) ? PJ : PI, qr, qc, lane, cacc);  // Q_I Q_J^T
-    const bool next_diag = (I == b + 1);  // then J == I as well
+    const bool fac = (tk.flags & kTaskFactor) != 0;
     const int li = lane & 15, lk = lane >> 4;
 #pragma unroll
     for (int x = 0; x < 2; x++)
@@ -313,62 +316,53 @@ __global__ __launch_bounds__(256) void k_chol_step(double *__restrict__ A, doubl
 #pragma unroll
         for (int v = 0; v < 4; v++) {
           const int r = qr + 16 * x + lk + 4 * v, c = qc + 16 * y + li;
-          const int64_t gi = (int64_t)I * kNB + r, gj = (int64_t)J * kNB + c;
           const double a = CT[r * kLd + c] - cacc[x][y][v];
-          if (next_diag) DV[r * kLd + c] = (gi < n && gj < n) ? a : (r == c ? 1.0 : 0.0);
-          else if (gi < n && gj < n) A[gi * n + gj] = a;
+          if (fac) DV[r * kLd + c] = a;
+          else A[(gi + r) * np + gj + c] = a;
         }
-    if (!next_diag) return;
-    fblk = b + 1;
+    if (!fac) return;
   }
   __syncthreads();
   factor_tile(DV, PJ, Ys, wbuf, info);
-  store_diag(DV, PJ, Lf, Dinv, n, fblk);
+  store_diag(DV, PJ, Lf, Dinv, np, tk.I);
 }
 
 // ---------------------------------------------------------------------------
-// W = L^{-1}, WT = W^T.  Workgroup s owns columns [16s, 16s+16), starting at
-// block row J = 16s/64 (W_J = columns of L_JJ^{-1}); below it
-//   W_I = -L_II^{-1} sum_{K=max(J, I-kband)}^{I-1} L_IK W_K,
-// with the last kband+1 block rows of the strip kept in an LDS ring (for
-// bands too wide for LDS, nring = 0, they are read back from W).  The
-// L_IK tiles are staged through LDS with every thread's loads of a tile in
-// flight together, the next tile loading while the current one is used;
-// the rows of L_II^{-1} a wave needs are loaded at the start of the step.
-__global__ __launch_bounds__(256) void k_linv_strip(const double *__restrict__ Lf,
-                                                    const double *__restrict__ Dinv, int n,
-                                                    int kband, int nring,
-                                                    double *__restrict__ W,
-                                                    double *__restrict__ WT) {
-  extern __shared__ double lds[];
-  double *ring = lds;                               // [nring][64][16]
-  double *Bx = ring + (size_t)nring * kNB * kStrip;  // [64][17]
-  double *Lt = Bx + kNB * 17;                        // [2][64][kLd]
+// W = L^{-1}, WT = W^T by 16-column strips.  The strip [16s, 16s+16) of tile
+// column J starts with W_JJ = L_JJ^{-1} and visits the tile rows where column
+// J of W is non-zero, in increasing order:
+//   W_I = -L_II^{-1} sum_{K in ks(I)} L_IK W_K,
+// ks(I) the strip's earlier rows with L_IK != 0 (row and K lists from the
+// elimination schedule).  W_K is read back from W (written by this
+// workgroup before a barrier); the L_IK tiles are staged through LDS with the
+// next tile's loads in flight during the current product; the rows of
+// L_II^{-1} a wave needs are loaded at the start of the row.
+__global__ __launch_bounds__(256) void k_linv_seq(const double *__restrict__ Lf,
+                                                  const double *__restrict__ Dinv, int np,
+                                                  const int2 *__restrict__ colinfo,
+                                                  const int4 *__restrict__ rows,
+                                                  const int *__restrict__ klist,
+                                                  double *__restrict__ W,
+                                                  double *__restrict__ WT) {
+  __shared__ double Bx[kNB * 17];
+  __shared__ double Lt[2 * kNB * kLd];
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const int li = lane & 15, lk = lane >> 4;
-  const int nblk = (n + kNB - 1) / kNB;
   const int c0 = blockIdx.x * kStrip;
   const int J = c0 / kNB, cofs = c0 - J * kNB;
   const int gc = c0 + li;
-  {
-    double *slot = nring ? ring + (size_t)(J % nring) * kNB * kStrip : nullptr;
-    for (int e = tid; e < kNB * kStrip; e += 256) {
-      const int r = e >> 4, j = e & 15;
-      const double v = Dinv[(int64_t)J * kNB * kNB + r * kNB + cofs + j];
-      if (nring) slot[e] = v;
-      const int64_t gr = (int64_t)J * kNB + r, gcol = c0 + j;
-      if (gr < n && gcol < n) {
-        W[gr * n + gcol] = v;
-        WT[gcol * n + gr] = v;
-      }
-    }
+  for (int e = tid; e < kNB * kStrip; e += 256) {
+    const int r = e >> 4, j = e & 15;
+    const double v = Dinv[(int64_t)J * kNB * kNB + r * kNB + cofs + j];
+    const int64_t gr = (int64_t)J * kNB + r;
+    W[gr * np + c0 + j] = v;
+    WT[(int64_t)(c0 + j) * np + gr] = v;
   }
   auto load_tile = [&](int I, int K, double (&reg)[16]) {
 #pragma unroll
     for (int q = 0; q < 16; q++) {
-      const int e = tid + 256 * q, r = e >> 6, c = e & 63;
-      const int64_t row = (int64_t)I * kNB + r;
-      reg[q] = row < n ? Lf[row * n + (int64_t)K * kNB + c] : 0.0;
+      const int e = tid + 256 * q;
+      reg[q] = Lf[((int64_t)I * kNB + (e >> 6)) * np + (int64_t)K * kNB + (e & 63)];
     }
   };
   auto store_tile = [&](double *dst, const double (&reg)[16]) {
@@ -379,42 +373,31 @@ __global__ __launch_bounds__(256) void k_linv_strip(const double *__restrict__ L
     }
   };
   __syncthreads();
-  for (int I = J + 1; I < nblk; I++) {
-    const int Kmin = max(J, I - kband);
-    // rows 16w.. of L_II^{-1}: columns k < 16(w+1) are non-zero
+  const int2 ci = colinfo[J];
+  for (int q = 0; q < ci.y; q++) {
+    const int4 rw = rows[ci.x + q];  // (I, offset into klist, count, -)
+    const int I = rw.x;
     const double *D = Dinv + (int64_t)I * kNB * kNB;
-    double dreg[16];
+    double dreg[16];  // rows 16w.. of L_II^{-1}: columns k < 16(w+1) are non-zero
 #pragma unroll
-    for (int q = 0; q < 16; q++)
-      dreg[q] = (4 * q < 16 * (w + 1)) ? D[(16 * w + li) * kNB + 4 * q + lk] : 0.0;
+    for (int t = 0; t < 16; t++)
+      dreg[t] = (4 * t < 16 * (w + 1)) ? D[(16 * w + li) * kNB + 4 * t + lk] : 0.0;
     double treg[16];
-    load_tile(I, Kmin, treg);
+    load_tile(I, klist[rw.y], treg);
     store_tile(Lt, treg);
     __syncthreads();
     double4_t acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
-    for (int K = Kmin; K < I; K++) {
-      const double *cur = Lt + (size_t)((K - Kmin) & 1) * kNB * kLd;
-      double *nxt = Lt + (size_t)(((K - Kmin) & 1) ^ 1) * kNB * kLd;
-      if (K + 1 < I) load_tile(I, K + 1, treg);
-      if (nring) {
-        const double *slot = ring + (size_t)(K % nring) * kNB * kStrip;
+    for (int t = 0; t < rw.z; t++) {
+      const double *cur = Lt + (size_t)(t & 1) * kNB * kLd;
+      double *nxt = Lt + (size_t)((t & 1) ^ 1) * kNB * kLd;
+      if (t + 1 < rw.z) load_tile(I, klist[rw.y + t + 1], treg);
+      const double *wk = W + (int64_t)klist[rw.y + t] * kNB * np + gc;
 #pragma unroll
-        for (int k = 0; k < kNB; k += 8) {
-          acc0 = mfma4(cur[(16 * w + li) * kLd + k + lk], slot[(k + lk) * kStrip + li], acc0);
-          acc1 = mfma4(cur[(16 * w + li) * kLd + k + 4 + lk], slot[(k + 4 + lk) * kStrip + li],
-                       acc1);
-        }
-      } else {
-        const double *wk = W + (int64_t)K * kNB * n + gc;
-#pragma unroll
-        for (int k = 0; k < kNB; k += 8) {
-          const double b0 = gc < n ? wk[(int64_t)(k + lk) * n] : 0.0;
-          const double b1 = gc < n ? wk[(int64_t)(k + 4 + lk) * n] : 0.0;
-          acc0 = mfma4(cur[(16 * w + li) * kLd + k + lk], b0, acc0);
-          acc1 = mfma4(cur[(16 * w + li) * kLd + k + 4 + lk], b1, acc1);
-        }
+      for (int k = 0; k < kNB; k += 8) {
+        acc0 = mfma4(cur[(16 * w + li) * kLd + k + lk], wk[(int64_t)(k + lk) * np], acc0);
+        acc1 = mfma4(cur[(16 * w + li) * kLd + k + 4 + lk], wk[(int64_t)(k + 4 + lk) * np], acc1);
       }
-      if (K + 1 < I) store_tile(nxt, treg);
+      if (t + 1 < rw.z) store_tile(nxt, treg);
       __syncthreads();
     }
 #pragma unroll
@@ -422,28 +405,24 @@ __global__ __launch_bounds__(256) void k_linv_strip(const double *__restrict__ L
     __syncthreads();
     double4_t o = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int q = 0; q < 16; q++)
-      if (4 * q < 16 * (w + 1)) o = mfma4(dreg[q], Bx[(4 * q + lk) * 17 + li], o);
-    double *slot = nring ? ring + (size_t)(I % nring) * kNB * kStrip : nullptr;
+    for (int t = 0; t < 16; t++)
+      if (4 * t < 16 * (w + 1)) o = mfma4(dreg[t], Bx[(4 * t + lk) * 17 + li], o);
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-      const int r = 16 * w + lk + 4 * v;
+      const int64_t gr = (int64_t)I * kNB + 16 * w + lk + 4 * v;
       const double val = -o[v];
-      if (nring) slot[r * kStrip + li] = val;
-      const int64_t gr = (int64_t)I * kNB + r;
-      if (gr < n && gc < n) {
-        W[gr * n + gc] = val;
-        WT[(int64_t)gc * n + gr] = val;
-      }
+      W[gr * np + gc] = val;
+      WT[(int64_t)gc * np + gr] = val;
     }
     __syncthreads();
   }
 }
 
 // C = WT W restricted to m >= max(i, j) (the explicit inverse A^{-1}),
-// 64x64 tiles, k staged through LDS in chunks of 16.
+// 64x64 tiles, k staged through LDS in chunks of 16.  C is n x n; W and WT
+// have row stride ld.
 __global__ __launch_bounds__(256) void k_gemm_wtw(const double *__restrict__ WT,
-                                                  const double *__restrict__ W, int n,
+                                                  const double *__restrict__ W, int n, int ld,
                                                   double *__restrict__ C) {
   __shared__ double As[kNB][17];
   __shared__ double Bs[16][kNB + 1];
@@ -458,9 +437,9 @@ __global__ __launch_bounds__(256) void k_gemm_wtw(const double *__restrict__ WT,
   for (int m0 = std::max(i0, j0); m0 < n; m0 += 16) {
     for (int e = tid; e < kNB * 16; e += 256) {
       const int r = e >> 4, k = e & 15;
-      As[r][k] = (i0 + r < n && m0 + k < n) ? WT[(int64_t)(i0 + r) * n + m0 + k] : 0.0;
+      As[r][k] = (i0 + r < n && m0 + k < n) ? WT[(int64_t)(i0 + r) * ld + m0 + k] : 0.0;
       const int kk = e >> 6, c = e & 63;
-      Bs[kk][c] = (m0 + kk < n && j0 + c < n) ? W[(int64_t)(m0 + kk) * n + j0 + c] : 0.0;
+      Bs[kk][c] = (m0 + kk < n && j0 + c < n) ? W[(int64_t)(m0 + kk) * ld + j0 + c] : 0.0;
     }
     __syncthreads();
 #pragma unroll
@@ -490,68 +469,92 @@ __global__ __launch_bounds__(256) void k_gemm_wtw(const double *__restrict__ WT,
 
 // ---------------------------------------------------------------------------
 // host side
-static size_t strip_lds_bytes(int nring) {
-  return sizeof(double) * ((size_t)nring * kNB * kStrip + (size_t)kNB * 17 + 2 * (size_t)kNB * kLd);
+// A[s][s] = 1 on the padding indices (ipos[s] < 0)
+__global__ void k_pad_identity(int np, const int *__restrict__ ipos, double *__restrict__ A) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < np && ipos[s] < 0) A[(int64_t)s * np + s] = 1.0;
 }
 
-static void *zalloc(size_t bytes) { return dalloc(bytes); }
-
-bool CoarseFactor::allocate(int n_, int bw_, bool f32) {
-  n = n_;
-  bw = std::min(bw_, n_ - 1);
-  if (bw < 0) bw = 0;
-  nblk = (n + kNB - 1) / kNB;
-  kband = (bw + kNB - 1) / kNB;
-  nring = std::min(kband + 1, nblk);
-  if (strip_lds_bytes(nring) > 158 * 1024) nring = 0;  // band rows read back from W
-  const size_t nn = (size_t)n * n;
-  A = (double *)zalloc(sizeof(double) * nn);
-  L = (double *)zalloc(sizeof(double) * nn);
-  W = (double *)zalloc(sizeof(double) * nn);
-  WT = (double *)zalloc(sizeof(double) * nn);
-  Dinv = (double *)zalloc(sizeof(double) * (size_t)nblk * kNB * kNB);
-  info = (int *)zalloc(sizeof(int) * 4);
-  cw64 = (double *)zalloc(sizeof(double) * 2 * (size_t)n);
-  if (f32) {
-    W32 = (float *)zalloc(sizeof(float) * nn);
-    WT32 = (float *)zalloc(sizeof(float) * nn);
-    cw32 = (float *)zalloc(sizeof(float) * 2 * (size_t)n);
+bool CoarseFactor::allocate(int n_, int bw_, bool f32, int planes) {
+  CoarseSchedule cs;
+  if (!build_coarse_schedule(n_, bw_, planes, cs)) {
+    set_error(TOP3D_ERR_ARG, "coarse schedule: %s", cs.error.c_str());
+    return false;
   }
-  return A && L && W && WT && Dinv && info && cw64 && (!f32 || (W32 && WT32 && cw32));
+  n = n_;
+  bw = std::max(0, std::min(bw_, n_ - 1));
+  np = cs.np;
+  nblk = cs.nblk;
+  std::vector<int> hipos(np, -1);
+  for (int d = 0; d < n; d++) hipos[cs.pos[d]] = d;
+  const size_t nn = (size_t)np * np;
+  A = (double *)dalloc(sizeof(double) * nn);
+  L = (double *)dalloc(sizeof(double) * nn);
+  W = (double *)dalloc(sizeof(double) * nn);
+  WT = (double *)dalloc(sizeof(double) * nn);
+  Dinv = (double *)dalloc(sizeof(double) * (size_t)nblk * kNB * kNB);
+  info = (int *)dalloc(sizeof(int) * 4);
+  cw64 = (double *)dalloc(sizeof(double) * 2 * (size_t)np);
+  pos = (int *)dalloc(sizeof(int) * (size_t)std::max(n, 1));
+  ipos = (int *)dalloc(sizeof(int) * (size_t)np);
+  ntask = (int)cs.tasks.size();
+  dtasks = (CholTask *)dalloc(sizeof(CholTask) * std::max<size_t>(1, cs.tasks.size()));
+  dcolinfo = (int2 *)dalloc(sizeof(int) * cs.colinfo.size());
+  drows = (int4 *)dalloc(sizeof(int) * std::max<size_t>(4, cs.rows.size()));
+  dklist = (int *)dalloc(sizeof(int) * std::max<size_t>(1, cs.klist.size()));
+  if (f32) {
+    W32 = (float *)dalloc(sizeof(float) * nn);
+    WT32 = (float *)dalloc(sizeof(float) * nn);
+    cw32 = (float *)dalloc(sizeof(float) * 2 * (size_t)np);
+  }
+  if (!(A && L && W && WT && Dinv && info && cw64 && pos && ipos && dtasks && dcolinfo &&
+        drows && dklist && (!f32 || (W32 && WT32 && cw32))))
+    return false;
+  launch_first = cs.launch_first;
+  launch_count = cs.launch_count;
+  hipStream_t s = stream();
+  auto up = [&](void *dst, const void *src, size_t bytes) {
+    return bytes == 0 || T3D_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+  };
+  return up(pos, cs.pos.data(), sizeof(int) * n) && up(ipos, hipos.data(), sizeof(int) * np) &&
+         up(dtasks, cs.tasks.data(), sizeof(CholTask) * cs.tasks.size()) &&
+         up(dcolinfo, cs.colinfo.data(), sizeof(int) * cs.colinfo.size()) &&
+         up(drows, cs.rows.data(), sizeof(int) * cs.rows.size()) &&
+         up(dklist, cs.klist.data(), sizeof(int) * cs.klist.size()) &&
+         T3D_HIP(hipStreamSynchronize(s));
 }
 
 void CoarseFactor::release() {
   for (void *p : {(void *)A, (void *)L, (void *)W, (void *)WT, (void *)Dinv, (void *)info,
-                  (void *)cw64, (void *)W32, (void *)WT32, (void *)cw32})
+                  (void *)cw64, (void *)W32, (void *)WT32, (void *)cw32, (void *)pos,
+                  (void *)ipos, (void *)dtasks, (void *)dcolinfo, (void *)drows,
+                  (void *)dklist})
     if (p) dfree(p);
   A = L = W = WT = Dinv = cw64 = nullptr;
   W32 = WT32 = cw32 = nullptr;
-  info = nullptr;
+  info = pos = ipos = dklist = nullptr;
+  dtasks = nullptr;
+  dcolinfo = nullptr;
+  drows = nullptr;
+}
+
+void CoarseFactor::begin_assembly() {
+  launch_fill<double>((int64_t)np * np, A, 0.0);
+  hipLaunchKernelGGL(k_pad_identity, dim3((np + 255) / 256), dim3(256), 0, stream(), np, ipos,
+                     A);
 }
 
 int CoarseFactor::factor() {
   hipStream_t s = stream();
   hipMemsetAsync(info, 0, sizeof(int), s);
-  hipLaunchKernelGGL(k_chol_step, dim3(1), dim3(256), 0, s, A, L, Dinv, n, -1, info);
-  for (int b = 0; b + 1 < nblk; b++) {
-    const int k0 = b * kNB;
-    const int t1 = std::min(nblk - 1, (k0 + kNB - 1 + bw) / kNB);
-    const int ntile = t1 - b;
-    hipLaunchKernelGGL(k_chol_step, dim3(ntile * (ntile + 1) / 2), dim3(256), 0, s, A, L, Dinv,
-                       n, b, info);
-  }
-  const size_t lds = strip_lds_bytes(nring);
-  static bool attr = false;
-  if (!attr) {
-    hipFuncSetAttribute((const void *)k_linv_strip, hipFuncAttributeMaxDynamicSharedMemorySize,
-                        158 * 1024);
-    attr = true;
-  }
-  hipLaunchKernelGGL(k_linv_strip, dim3((n + kStrip - 1) / kStrip), dim3(256), lds, s, L, Dinv,
-                     n, kband, nring, W, WT);
+  for (size_t q = 0; q < launch_first.size(); q++)
+    hipLaunchKernelGGL(k_chol_task, dim3(launch_count[q]), dim3(256), 0, s, A, L, Dinv, np,
+                       dtasks + launch_first[q], info);
+  hipLaunchKernelGGL(k_linv_seq, dim3(np / kStrip), dim3(256), 0, s, L, Dinv, np, dcolinfo,
+                     drows, dklist, W, WT);
   if (W32) {
-    launch_convert<float, double>((int64_t)n * n, W, W32);
-    launch_convert<float, double>((int64_t)n * n, WT, WT32);
+    launch_convert<float, double>((int64_t)np * np, W, W32);
+    launch_convert<float, double>((int64_t)np * np, WT, WT32);
   }
   int hinfo = 0;
   hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s);
@@ -564,18 +567,20 @@ int CoarseFactor::factor() {
 }
 
 // ---------------------------------------------------------------------------
-// Coarsest solve z = W^T (W r): gather the interior DOFs, lower-triangular
-// GEMV, upper-triangular GEMV scattered back to the halo layout (the halo of
-// z stays zero).  One workgroup per row, FP64 accumulation.
+// Coarsest solve z = W^T (W r) in the factor's storage order: gather the
+// interior DOFs to their storage positions, lower-triangular GEMV,
+// upper-triangular GEMV scattered back to the halo layout (the halo of z
+// stays zero).  One workgroup per row, FP64 accumulation.
 template <typename T>
 __global__ void k_gather_interior(int nx, int ny, int nz, int wy, int wz,
-                                  const T *__restrict__ in, T *__restrict__ out) {
+                                  const int *__restrict__ pos, const T *__restrict__ in,
+                                  T *__restrict__ out) {
   const int64_t nint = (int64_t)nx * ny * nz;
   const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (d >= 3 * nint) return;
   const int64_t p = d / 3;
   const int j = (int)(p % ny), k = (int)((p / ny) % nz), i = (int)(p / ((int64_t)ny * nz));
-  out[d] = in[3 * (((int64_t)(i + 1) * wz + (k + 1)) * wy + (j + 1)) + d % 3];
+  out[pos[d]] = in[3 * (((int64_t)(i + 1) * wz + (k + 1)) * wy + (j + 1)) + d % 3];
 }
 
 // One 256-thread workgroup per row: the row (up to n values) is read by all
@@ -602,6 +607,7 @@ __global__ __launch_bounds__(256) void k_trmv_lower(const T *__restrict__ Wm, in
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_trmv_upper_scatter(int nx, int ny, int nz, int wy, int wz,
+                                                            const int *__restrict__ ipos,
                                                             const T *__restrict__ WTm, int n,
                                                             const T *__restrict__ y,
                                                             T *__restrict__ out) {
@@ -610,25 +616,26 @@ __global__ __launch_bounds__(256) void k_trmv_upper_scatter(int nx, int ny, int 
   double s = 0.0;
   for (int c = row + threadIdx.x; c < n; c += 256) s += (double)a[c] * (double)y[c];
   s = row_block_sum(s);
-  if (threadIdx.x == 0) {
-    const int64_t p = row / 3;
+  const int d = ipos[row];
+  if (threadIdx.x == 0 && d >= 0) {
+    const int64_t p = d / 3;
     const int j = (int)(p % ny), k = (int)((p / ny) % nz), i = (int)(p / ((int64_t)ny * nz));
-    out[3 * (((int64_t)(i + 1) * wz + (k + 1)) * wy + (j + 1)) + row % 3] = (T)s;
+    out[3 * (((int64_t)(i + 1) * wz + (k + 1)) * wy + (j + 1)) + d % 3] = (T)s;
   }
 }
 
 template <typename T>
 void launch_coarse_apply(const DevGrid &g, int l, const CoarseFactor &cf, const T *in, T *out) {
   const Level &L = g.lv[l];
-  const int n = cf.n;
+  const int n = cf.n, np = cf.np;
   const T *Wm = cf.w<T>(), *WTm = cf.wt<T>();
-  T *x = cf.cw<T>(), *y = x + n;
+  T *x = cf.cw<T>(), *y = x + np;  // padding entries of x stay zero
   const int bs = 256;
   hipLaunchKernelGGL((k_gather_interior<T>), dim3((n + bs - 1) / bs), dim3(bs), 0, stream(),
-                     L.nx, L.ny, L.nz, L.wy, L.wz, in, x);
-  hipLaunchKernelGGL((k_trmv_lower<T>), dim3(n), dim3(256), 0, stream(), Wm, n, x, y);
-  hipLaunchKernelGGL((k_trmv_upper_scatter<T>), dim3(n), dim3(256), 0, stream(), L.nx,
-                     L.ny, L.nz, L.wy, L.wz, WTm, n, y, out);
+                     L.nx, L.ny, L.nz, L.wy, L.wz, cf.pos, in, x);
+  hipLaunchKernelGGL((k_trmv_lower<T>), dim3(np), dim3(256), 0, stream(), Wm, np, x, y);
+  hipLaunchKernelGGL((k_trmv_upper_scatter<T>), dim3(np), dim3(256), 0, stream(), L.nx,
+                     L.ny, L.nz, L.wy, L.wz, cf.ipos, WTm, np, y, out);
 }
 
 template void launch_coarse_apply<double>(const DevGrid &, int, const CoarseFactor &,
@@ -647,16 +654,18 @@ extern "C" int top3d_dense_spd_inverse(double *A_dev, int n, int bw) {
     return TOP3D_ERR_ARG;
   }
   CoarseFactor cf;
-  if (!cf.allocate(n, bw, false)) {
+  if (!cf.allocate(n, bw, false, 0)) {
     cf.release();
     return top3d_last_error() ? top3d_last_error() : TOP3D_ERR_HIP;
   }
-  hipMemcpyAsync(cf.A, A_dev, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToDevice, stream());
+  cf.begin_assembly();
+  hipMemcpy2DAsync(cf.A, sizeof(double) * cf.np, A_dev, sizeof(double) * n, sizeof(double) * n,
+                   n, hipMemcpyDeviceToDevice, stream());
   int rc = cf.factor();
   if (rc == TOP3D_OK) {
     const int nblk = (n + kNB - 1) / kNB;
     hipLaunchKernelGGL(k_gemm_wtw, dim3(nblk * nblk), dim3(256), 0, stream(), cf.WT, cf.W, n,
-                       A_dev);
+                       cf.np, A_dev);
     if (!T3D_HIP(hipStreamSynchronize(stream()))) rc = TOP3D_ERR_HIP;
   }
   cf.release();

@@ -31,7 +31,9 @@
 
 namespace t3d {
 
-enum { kModeApply = 0, kModeJacobi = 1, kModeResidual = 2 };
+// kModeJacobiDiag: Jacobi sweep whose inverse diagonal the kernel forms from
+// the element moduli it already reads (fine level, solver only)
+enum { kModeApply = 0, kModeJacobi = 1, kModeResidual = 2, kModeJacobiDiag = 3 };
 
 // position bits (x, y, z) of the reference's local node n (stencil_utility.h:19-26)
 static constexpr int kPos[8][3] = {{0, 1, 0}, {1, 1, 0}, {1, 0, 0}, {0, 0, 0},
@@ -88,6 +90,10 @@ bool hadamard_blocks(const double *KE, double M[8][3][3]) {
 template <typename T>
 struct Blocks {
   T m[8][3][3];
+  // 1 / KE[3a+c][3a+c]: the diagonal of KE is the same for all 8 nodes, so
+  // the assembled diagonal at a node is KE[c][c] times the sum of the moduli
+  // of its elements
+  T ikd[3];
 };
 
 // The element transform h[q][c] = sum_p (-1)^{q.p} u[p][c] (p, q in {0,1}^3,
@@ -320,8 +326,11 @@ __device__ __forceinline__ void tile_sweep_body(
     const T *__restrict__ in, T *__restrict__ out, const T *__restrict__ b,
     const T *__restrict__ invD, T omega, const uint8_t *__restrict__ fix,
     double *__restrict__ dotp) {
+  static_assert(LVL == 0 || MODE != kModeJacobiDiag, "level 1 reads its inverse diagonal");
+  constexpr bool kDiag = MODE == kModeJacobiDiag;
+  constexpr int XS = kDiag ? 10 : 9;  // LDS values per thread and buffer
   extern __shared__ unsigned char smem_raw[];
-  T *xch = reinterpret_cast<T *>(smem_raw);  // [2][nthreads][9]
+  T *xch = reinterpret_cast<T *>(smem_raw);  // [2][nthreads][XS]
   const int nthr = a.TY * a.TZ;
   const int t = threadIdx.x;
   const bool active = t < nthr;
@@ -345,6 +354,8 @@ __device__ __forceinline__ void tile_sweep_body(
   const uint32_t n00 = (uint32_t)k * a.wy + j;  // node (j, k) within a plane
   const uint32_t noff[2][2] = {{0u, (uint32_t)a.wy}, {1u, (uint32_t)a.wy + 1u}};  // [py][pz]
   double dacc = 0.0;  // apply mode with dotp: partial sum of in . out
+  // kDiag: moduli of the 4 elements of the previous slab at the owned node
+  T es_prev = 0;
 
   // transformed face of node plane i (zero outside the grid) and the raw
   // values of its node (j+1, k+1)
@@ -397,10 +408,20 @@ __device__ __forceinline__ void tile_sweep_body(
         h[2 * f + 1][c] = Fc[f][c] - Fn[f][c];
       }
     T g[8][3];
+    T Ee = 0;
     if constexpr (LVL == 0) {
-      T Ee = 0;
       if (elem_ok && i <= a.nelx + 1) Ee = E[((uint32_t)i * a.fez + k) * a.few + j];
       element_core<T>(B, Ee, h, g);
+      if constexpr (kDiag) {
+        if (i < i0 && own_ok) {  // first slab: the neighbours' moduli from memory
+          const uint32_t eb = ((uint32_t)i * a.fez + k) * a.few + j;
+          T es = Ee;
+          if (!last_y) es += E[eb + 1];
+          if (!last_z) es += E[eb + a.few];
+          if (!last_y && !last_z) es += E[eb + a.few + 1];
+          es_prev = es;
+        }
+      }
     } else {
       T Eo[8];
 #pragma unroll
@@ -446,29 +467,34 @@ __device__ __forceinline__ void tile_sweep_body(
 #pragma unroll
         for (int c = 0; c < 3; c++) totT[f][c] = (g[2 * f][c] + g[2 * f + 1][c]) + pendT[f][c];
       face_iwht<T>(totT, tot);
-      T *buf = xch + (size_t)(i & 1) * nthr * 9;
+      T *buf = xch + (size_t)(i & 1) * nthr * XS;
       if (active) {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-          buf[9 * t + c] = tot[0][1][c];      // node (j, k+1)  -> thread (ty-1, tz)
-          buf[9 * t + 3 + c] = tot[1][0][c];  // node (j+1, k)  -> thread (ty, tz-1)
-          buf[9 * t + 6 + c] = tot[0][0][c];  // node (j, k)    -> thread (ty-1, tz-1)
+          buf[XS * t + c] = tot[0][1][c];      // node (j, k+1)  -> thread (ty-1, tz)
+          buf[XS * t + 3 + c] = tot[1][0][c];  // node (j+1, k)  -> thread (ty, tz-1)
+          buf[XS * t + 6 + c] = tot[0][0][c];  // node (j, k)    -> thread (ty-1, tz-1)
         }
+        if constexpr (kDiag) buf[XS * t + 9] = Ee;
       }
       __syncthreads();
       if (own_ok) {
         T kin[3] = {tot[1][1][0], tot[1][1][1], tot[1][1][2]};
+        T es = Ee;  // kDiag: moduli of this slab's 4 elements at the node
         if (!last_y) {
 #pragma unroll
-          for (int c = 0; c < 3; c++) kin[c] += buf[9 * (t + 1) + c];
+          for (int c = 0; c < 3; c++) kin[c] += buf[XS * (t + 1) + c];
+          if constexpr (kDiag) es += buf[XS * (t + 1) + 9];
         }
         if (!last_z) {
 #pragma unroll
-          for (int c = 0; c < 3; c++) kin[c] += buf[9 * (t + a.TY) + 3 + c];
+          for (int c = 0; c < 3; c++) kin[c] += buf[XS * (t + a.TY) + 3 + c];
+          if constexpr (kDiag) es += buf[XS * (t + a.TY) + 9];
         }
         if (!last_y && !last_z) {
 #pragma unroll
-          for (int c = 0; c < 3; c++) kin[c] += buf[9 * (t + a.TY + 1) + 6 + c];
+          for (int c = 0; c < 3; c++) kin[c] += buf[XS * (t + a.TY + 1) + 6 + c];
+          if constexpr (kDiag) es += buf[XS * (t + a.TY + 1) + 9];
         }
         const uint32_t nd = (uint32_t)i * sx + (uint32_t)(k + 1) * a.wy + (uint32_t)(j + 1);
         const uint32_t d0 = 3u * nd;
@@ -481,7 +507,13 @@ __device__ __forceinline__ void tile_sweep_body(
           bb[1] = pb[1];
           bb[2] = pb[2];
         }
-        if (MODE == kModeJacobi) {
+        if constexpr (kDiag) {
+          // diagonal KE[c][c] (es_prev + es); identity rows at fixed DOFs
+          const T rs = T(1) / (es_prev + es);
+          es_prev = es;
+#pragma unroll
+          for (int c = 0; c < 3; c++) dd[c] = ((m >> c) & 1) ? T(1) : rs * B.ikd[c];
+        } else if (MODE == kModeJacobi) {
           const T *pd = invD + d0;
           dd[0] = pd[0];
           dd[1] = pd[1];
@@ -503,7 +535,7 @@ __device__ __forceinline__ void tile_sweep_body(
               const T pv = (BCSYM && ((m >> c) & 1)) ? pin[c] : uc;
               dacc += (double)pv * (double)o[c];
             }
-          } else if (MODE == kModeJacobi) {
+          } else if (MODE == kModeJacobi || kDiag) {
             const T uo = BCSYM ? pin[c] : uc;
             o[c] = uo + omega * dd[c] * (bb[c] - kv);
           } else {
@@ -571,6 +603,7 @@ __global__ __launch_bounds__(512) void k_tile_sweep_f32(
 // host side
 struct HostCoef {
   bool done = false, ok0 = false, ok1 = false;
+  bool diag0 = false;  // KE_0 has one diagonal value per component
   Blocks<double> b64;
   Blocks<float> b32;
   K1Coef<double> k64;
@@ -593,6 +626,17 @@ static void init_coef() {
           g_coef.b64.m[s][c][d] = M[s][c][d];
           g_coef.b32.m[s][c][d] = (float)M[s][c][d];
         }
+  if (g_coef.ok0) {
+    g_coef.diag0 = true;
+    for (int c = 0; c < 3; c++) {
+      const double d = KE[24 * c + c];
+      for (int a = 1; a < 8; a++)
+        if (std::fabs(KE[24 * (3 * a + c) + 3 * a + c] - d) > 1e-14 * std::fabs(d))
+          g_coef.diag0 = false;
+      g_coef.b64.ikd[c] = 1.0 / d;
+      g_coef.b32.ikd[c] = (float)(1.0 / d);
+    }
+  }
   double coef[kK1Nnz];
   g_coef.ok1 = level1_hadamard(KE1.data(), coef);
   if (g_coef.ok1)
@@ -618,12 +662,17 @@ bool tile_level1_available() {
   return g_coef.ok1;
 }
 
+bool tile_fine_diag_available() {
+  init_coef();
+  return g_coef.ok0 && g_coef.diag0;
+}
+
 struct TileShape {
   int TY, TZ, NX, nty, ntz, ntx;
 };
 
 // Tile of TY x TZ element columns marching over NX node planes.  LDS holds
-// two exchange buffers of 9 values per thread (under 60 KiB).  Tuning
+// two exchange buffers of 9 or 10 values per thread (under 60 KiB).  Tuning
 // override: TOP3D_TILE_L<level>=TY,TZ,NX (NX = 0: from the occupancy).
 //
 // Shapes measured on MI355X at 256x128x128 (fine) / 128x64x64 (level 1):
@@ -635,7 +684,9 @@ struct TileShape {
 // rounds x (NX + 3), the per-workgroup cost in slab steps.
 static TileShape tile_shape(const Level &L, int lvl, int elem_bytes, int &nx_fixed) {
   TileShape s{};
-  const int lds_threads = 60 * 1024 / (18 * elem_bytes);
+  // LDS values per thread: 2 buffers of 9 (10 on the fine level, whose
+  // diagonal-forming sweep also exchanges a modulus)
+  const int lds_threads = 60 * 1024 / ((lvl == 0 ? 20 : 18) * elem_bytes);
   // launch bounds: 512 threads except level 1 in FP32 (1024)
   const int max_threads = std::min((elem_bytes == 4 && lvl == 1) ? 1024 : 512, lds_threads);
   struct Rule {
@@ -726,21 +777,32 @@ static int launch_tile(const DevGrid &g, int mode, const T *E, const T *in, T *o
               (long long)L.nodes());
     return 0;
   }
+  // the diagonal from the moduli needs the fine level and a KE whose
+  // diagonal is one value per component
+  if (mode == kModeJacobiDiag && !(LVL == 0 && g_coef.diag0)) mode = kModeJacobi;
+  const bool diag = mode == kModeJacobiDiag;
   int nx_fixed = 0;
   TileShape s = tile_shape(L, LVL, (int)sizeof(T), nx_fixed);
   const int nthr = s.TY * s.TZ;
   const int bs = (nthr + 63) / 64 * 64;
-  const size_t shm = sizeof(T) * 2 * 9 * (size_t)nthr;
-  static int cached_nthr = -1, cached_slots = 1;  // per instantiation
+  const size_t shm = sizeof(T) * 2 * (diag ? 10 : 9) * (size_t)nthr;
+  // per instantiation, [0] sweeps reading invD, [1] forming the diagonal
+  static int cached_nthr[2] = {-1, -1}, cached_slots[2] = {1, 1};
   constexpr bool kF32Fine = sizeof(T) == 4 && LVL == 0;
-  if (cached_nthr != nthr) {
+  if (cached_nthr[diag] != nthr) {
     const void *kfn;
-    if constexpr (kF32Fine) kfn = (const void *)k_tile_sweep_f32<kModeJacobi, false>;
-    else kfn = (const void *)k_tile_sweep<T, LVL, kModeJacobi, false>;
-    cached_slots = wg_per_cu(kfn, bs, shm);
-    cached_nthr = nthr;
+    if constexpr (kF32Fine)
+      kfn = diag ? (const void *)k_tile_sweep_f32<kModeJacobiDiag, false>
+                 : (const void *)k_tile_sweep_f32<kModeJacobi, false>;
+    else if constexpr (LVL == 0)
+      kfn = diag ? (const void *)k_tile_sweep<T, LVL, kModeJacobiDiag, false>
+                 : (const void *)k_tile_sweep<T, LVL, kModeJacobi, false>;
+    else
+      kfn = (const void *)k_tile_sweep<T, LVL, kModeJacobi, false>;
+    cached_slots[diag] = wg_per_cu(kfn, bs, shm);
+    cached_nthr[diag] = nthr;
   }
-  tile_x(s, L, nx_fixed, (int64_t)cached_slots * device_cus());
+  tile_x(s, L, nx_fixed, (int64_t)cached_slots[diag] * device_cus());
   TileArgs a;
   a.nelx = L.nelx;
   a.nely = L.nely;
@@ -775,11 +837,13 @@ static int launch_tile(const DevGrid &g, int mode, const T *E, const T *in, T *o
   if (bcsym) {
     if (mode == kModeApply) T3D_LAUNCH(kModeApply, true);
     else if (mode == kModeJacobi) T3D_LAUNCH(kModeJacobi, true);
-    else T3D_LAUNCH(kModeResidual, true);
+    else if (mode == kModeResidual) T3D_LAUNCH(kModeResidual, true);
+    else if constexpr (LVL == 0) T3D_LAUNCH(kModeJacobiDiag, true);
   } else {
     if (mode == kModeApply) T3D_LAUNCH(kModeApply, false);
     else if (mode == kModeJacobi) T3D_LAUNCH(kModeJacobi, false);
-    else T3D_LAUNCH(kModeResidual, false);
+    else if (mode == kModeResidual) T3D_LAUNCH(kModeResidual, false);
+    else if constexpr (LVL == 0) T3D_LAUNCH(kModeJacobiDiag, false);
   }
 #undef T3D_LAUNCH
   return dotp ? nwg : 0;
@@ -802,6 +866,56 @@ void launch_level1_tile(const DevGrid &g, int mode, const T *E, const T *in, T *
                         const T *b, const T *invD, T omega, int bcsym) {
   launch_tile<T, 1>(g, mode, E, in, out, b, invD, omega, bcsym);
 }
+
+// First pre-smoothing sweep of the fine level from z = 0, with the diagonal
+// formed from the moduli as in the kJacobiDiag sweep:
+//   out = w b / (KE[c][c] sum of the node's 8 element moduli)  (real nodes),
+//   out = w b at fixed DOFs, 0 on every other node.
+template <typename TB, typename T>
+__global__ __launch_bounds__(256) void k_jacobi_zero_fine(
+    int nx, int ny, int nz, int wy, int wz, int few, int fez, const TB *__restrict__ b,
+    const T *__restrict__ E, const uint8_t *__restrict__ fix, Blocks<T> B, T omega,
+    T *__restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y * blockDim.y + threadIdx.y;
+  const int i = blockIdx.z;
+  if (j >= wy || k >= wz) return;
+  const int64_t node = ((int64_t)i * wz + k) * wy + j;
+  T *po = out + 3 * node;
+  if (!(i >= 1 && i <= nx && j >= 1 && j <= ny && k >= 1 && k <= nz)) {
+    po[0] = po[1] = po[2] = T(0);
+    return;
+  }
+  // the node's elements: (i-1..i, j-1..j, k-1..k) in halo element indices
+  auto slab = [&](int ie) {
+    const int64_t e = ((int64_t)ie * fez + (k - 1)) * few + (j - 1);
+    return ((E[e] + E[e + 1]) + E[e + few]) + E[e + few + 1];
+  };
+  const T rs = T(1) / (slab(i - 1) + slab(i));
+  const uint8_t m = fix[node];
+  const TB *pb = b + 3 * node;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const T dd = ((m >> c) & 1) ? T(1) : rs * B.ikd[c];
+    po[c] = omega * dd * (T)pb[c];
+  }
+}
+
+template <typename TB, typename T>
+bool launch_fine_jacobi_zero(const DevGrid &g, const TB *b, const T *E, T omega, T *out) {
+  if (!tile_fine_diag_available()) return false;
+  const Level &L = g.lv[0];
+  Blocks<T> B;
+  if constexpr (sizeof(T) == 8) B = g_coef.b64; else B = g_coef.b32;
+  dim3 bs(64, 4, 1), gs((L.wy + 63) / 64, (L.wz + 3) / 4, L.wx);
+  hipLaunchKernelGGL((k_jacobi_zero_fine<TB, T>), gs, bs, 0, stream(), L.nx, L.ny, L.nz, L.wy,
+                     L.wz, g.ewy, g.ewz, b, E, g.fixmask[0], B, omega, out);
+  return true;
+}
+template bool launch_fine_jacobi_zero<double, double>(const DevGrid &, const double *,
+                                                      const double *, double, double *);
+template bool launch_fine_jacobi_zero<float, float>(const DevGrid &, const float *,
+                                                    const float *, float, float *);
 
 // rows j > nely+1 of every (i, k) set to zero, as the reference's operator
 // does for its padded y nodes (stencil_methods.c:407-418)

@@ -159,6 +159,8 @@ static bool use_node_kernel() {
   return v == 1;
 }
 
+bool fine_sweep_forms_diagonal() { return !use_node_kernel() && tile_fine_diag_available(); }
+
 template <typename T>
 void launch_fine(const DevGrid &g, int mode, const T *E, const T *in, T *out,
                  const T *b, const T *invD, T omega, int bcsym) {
@@ -166,6 +168,7 @@ void launch_fine(const DevGrid &g, int mode, const T *E, const T *in, T *out,
     launch_fine_tile<T>(g, mode, E, in, out, b, invD, omega, bcsym);
     return;
   }
+  if (mode == 3) mode = kJacobi;  // the node kernel reads invD
   const Level &L = g.lv[0];
   const LevelArgs a = g.args(0);
   const T *KE = nullptr;

@@ -72,7 +72,7 @@ bool SolverState::allocate(DevGrid *grid, int nlev) {
   const Level &C = g->lv[nl - 1];
   ncoarse = (int)(3 * C.interior_nodes());
   bandwidth = coarse_bandwidth(C);
-  if (!coarse.allocate(ncoarse, bandwidth, true)) return false;
+  if (!coarse.allocate(ncoarse, bandwidth, true, C.nx)) return false;
   scal = alloc_t<double>(16);
   partials = alloc_t<double>(1024 + 16);
   cgpart = alloc_t<double>(kPartRZ + kPartPQ + kPartRR);
@@ -113,7 +113,9 @@ void SolverState::release() {
 bool SolverState::setup(const double *x) {
   launch_young<double>(*g, x, E64);
   if (mixed) launch_convert<float, double>(g->nelem, E64, E32);
-  launch_invdiag_mf<double>(*g, 0, E64, invD64[0]);
+  // the fine Jacobi sweeps form their diagonal from E when they can
+  const bool fdiag = fine_sweep_forms_diagonal();
+  if (!fdiag) launch_invdiag_mf<double>(*g, 0, E64, invD64[0]);
   launch_invdiag_mf<double>(*g, 1, E64, invD64[1]);
   for (int l = 2; l < nl; l++) {
     if (l == 2)
@@ -123,10 +125,11 @@ bool SolverState::setup(const double *x) {
     launch_node_stencil(*g, l, Ke[l], S64[l], 1);
     launch_invdiag_stencil<double>(*g, l, S64[l], invD64[l]);
   }
-  launch_stencil_to_dense(*g, nl - 1, S64[nl - 1], coarse.A, ncoarse);
+  launch_stencil_to_dense(*g, nl - 1, S64[nl - 1], coarse);
   if (coarse.factor() != TOP3D_OK) return false;
   if (mixed) {
-    for (int l = 0; l < nl; l++) launch_convert<float, double>(g->lv[l].ndof, invD64[l], invD32[l]);
+    for (int l = fdiag ? 1 : 0; l < nl; l++)
+      launch_convert<float, double>(g->lv[l].ndof, invD64[l], invD32[l]);
     for (int l = 2; l < nl - 1; l++)
       launch_convert<float, double>(243 * g->lv[l].interior_nodes(), S64[l], S32[l]);
   }
@@ -147,13 +150,13 @@ void SolverState::op(int l, int mode, const T *in, T *out, const T *b) {
     }
     const bool rec = sw_ev.size() >= sw_used + 2;
     if (rec) hipEventRecord(sw_ev[sw_used], stream());
-    launch_fine<T>(*g, mode, young_t<T>(), in, out, b, invD, omega, bcsym);
+    launch_fine<T>(*g, 3, young_t<T>(), in, out, b, invD, omega, bcsym);
     if (rec) {
       hipEventRecord(sw_ev[sw_used + 1], stream());
       sw_used += 2;
     }
-  } else if (l == 0)
-    launch_fine<T>(*g, mode, young_t<T>(), in, out, b, invD, omega, bcsym);
+  } else if (l == 0)  // Jacobi sweeps form the diagonal from the moduli
+    launch_fine<T>(*g, mode == 1 ? 3 : mode, young_t<T>(), in, out, b, invD, omega, bcsym);
   else if (l == 1)
     launch_level1<T>(*g, mode, young_t<T>(), in, out, b, invD, omega, bcsym);
   else
@@ -167,7 +170,10 @@ template <typename T>
 T *SolverState::smooth_from_zero(int l, const T *b, T *zb, T *tb) {
   T *cur = (nswp % 2 == 1) ? zb : tb;
   T *oth = (cur == zb) ? tb : zb;
-  launch_jacobi_zero<T, T>(*g, l, b, invd<T>(l), (T)kOmega, cur);
+  if (l == 0 && fine_sweep_forms_diagonal())
+    launch_fine_jacobi_zero<T, T>(*g, b, young_t<T>(), (T)kOmega, cur);
+  else
+    launch_jacobi_zero<T, T>(*g, l, b, invd<T>(l), (T)kOmega, cur);
   for (int s = 1; s < nswp; s++) {
     op<T>(l, 1, cur, oth, b);
     std::swap(cur, oth);
@@ -396,10 +402,11 @@ int top3d_sweep_timing(struct SolverDataBuffer *data, int enable, double *avg_ms
   s->sweep_timing = enable != 0;
   if (avg_ms) *avg_ms = s->sw_count ? s->sw_ms / s->sw_count : 0.0;
   if (count) *count = s->sw_count;
-  if (bytes) {
+  if (bytes) {  // u, b (and invD unless formed from E) in; u out; E
     const DevGrid &g = *s->g;
     const double nn0 = (double)g.lv[0].interior_nodes(), ne0 = (double)g.lv[0].elements();
-    *bytes = s->mixed ? 48.0 * nn0 + 4.0 * ne0 : 96.0 * nn0 + 8.0 * ne0;
+    const double vec = fine_sweep_forms_diagonal() ? 3.0 : 4.0;
+    *bytes = s->mixed ? 12.0 * vec * nn0 + 4.0 * ne0 : 24.0 * vec * nn0 + 8.0 * ne0;
   }
   return TOP3D_OK;
 }
@@ -428,13 +435,13 @@ int top3d_time_kernel(struct SolverDataBuffer *data, int which, int reps, double
     switch (which) {
       case 0: {
         float *a = s->v32[0][SolverState::kZ], *b = s->t32[0];
-        launch_fine<float>(g, 1, s->E32, odd ? b : a, odd ? a : b, s->v32[0][SolverState::kD],
+        launch_fine<float>(g, 3, s->E32, odd ? b : a, odd ? a : b, s->v32[0][SolverState::kD],
                            s->invD32[0], (float)kOmega, s->bcsym);
         break;
       }
       case 1: {
         double *a = s->z, *b = s->t64[0];
-        launch_fine<double>(g, 1, s->E64, odd ? b : a, odd ? a : b, s->v64[0][SolverState::kD],
+        launch_fine<double>(g, 3, s->E64, odd ? b : a, odd ? a : b, s->v64[0][SolverState::kD],
                             s->invD64[0], kOmega, s->bcsym);
         break;
       }
@@ -468,9 +475,11 @@ int top3d_time_kernel(struct SolverDataBuffer *data, int which, int reps, double
   hipEventDestroy(e1);
   if (avg_ms) *avg_ms = ms / reps;
   if (bytes) {
+    // fine sweeps: u, b (and invD unless formed from E) in; u out; E
+    const double vec = fine_sweep_forms_diagonal() ? 3.0 : 4.0;
     switch (which) {
-      case 0: *bytes = 48.0 * nn0 + 4.0 * ne0; break;   // u, b, invD in; u out; E
-      case 1: *bytes = 96.0 * nn0 + 8.0 * ne0; break;
+      case 0: *bytes = 12.0 * vec * nn0 + 4.0 * ne0; break;
+      case 1: *bytes = 24.0 * vec * nn0 + 8.0 * ne0; break;
       case 2: *bytes = 48.0 * nn0 + 8.0 * ne0; break;   // p in, q out, E
       case 3: *bytes = 48.0 * nn1 + 4.0 * ne0; break;   // level-1 vectors, fine E
       default: *bytes = 96.0 * nn1 + 8.0 * ne0;

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/top3d_hip.h"
+#include "coarse_schedule.hpp"
 
 namespace t3d {
 
@@ -88,7 +89,9 @@ template <typename T>
 void launch_young_float_input(const DevGrid &g, const float *x, T *E);
 
 // fine level: mode 0 out = K in; 1 Jacobi sweep out = in + w invD (b - K in)
-// (interior nodes; other nodes copied from in); 2 residual out = b - K in
+// (interior nodes; other nodes copied from in); 2 residual out = b - K in;
+// 3 as 1 with invD the inverse diagonal of this operator, which the kernel
+// may form from E instead of reading it
 template <typename T>
 void launch_fine(const DevGrid &g, int mode, const T *E, const T *in, T *out,
                  const T *b, const T *invD, T omega, int bcsym);
@@ -99,6 +102,9 @@ template <typename T>
 void launch_level1_tile(const DevGrid &g, int mode, const T *E, const T *in, T *out,
                         const T *b, const T *invD, T omega, int bcsym);
 bool tile_level1_available();  // the level-1 element matrices have the mirror structure
+bool tile_fine_diag_available();  // the fine tile sweep can form its diagonal from E
+// launch_fine mode 3 forms the diagonal from E (no invD read)
+bool fine_sweep_forms_diagonal();
 // zero the padded y rows (j > nely+1) of a level-0 vector
 void launch_zero_pad_rows(const DevGrid &g, double *out);
 // level-1 matrix-free operator with fine moduli, same modes
@@ -109,6 +115,10 @@ void launch_level1(const DevGrid &g, int mode, const T *E, const T *in, T *out,
 template <typename T>
 void launch_stencil(const DevGrid &g, int l, int mode, const T *S, const T *in,
                     T *out, const T *b, const T *invD, T omega);
+// fine-level sweep from z = 0 with the diagonal formed from E; false if the
+// element matrix does not allow it
+template <typename TB, typename T>
+bool launch_fine_jacobi_zero(const DevGrid &g, const TB *b, const T *E, T omega, T *out);
 // first pre-smoothing sweep from z = 0:  out = w invD b on interior, else 0
 template <typename TB, typename T>
 void launch_jacobi_zero(const DevGrid &g, int l, const TB *b, const T *invD,
@@ -131,23 +141,42 @@ void launch_elem_galerkin(const DevGrid &g, int l, const double *Kchild,
                           double *Kparent);
 void launch_node_stencil(const DevGrid &g, int l, const double *Ke, double *S,
                          int bc_symmetric);
-void launch_stencil_to_dense(const DevGrid &g, int l, const double *S,
-                             double *A, int n);
+struct CoarseFactor;
+// assemble level l's stencil into cf.A (storage order, after begin_assembly)
+void launch_stencil_to_dense(const DevGrid &g, int l, const double *S, CoarseFactor &cf);
 
-// dense FP64 coarse solver: A = L L^T (banded), W = L^{-1}, WT = W^T, so
-// that A^{-1} = WT W; FP32 copies of W and WT for the mixed V-cycle
+// dense FP64 coarse solver: P A P^T = L L^T over 64x64 tiles in an
+// elimination order (nested dissection over the grid's x-planes, or banded),
+// W = L^{-1}, WT = W^T, so that A^{-1} = P^T WT W P; FP32 copies of W and WT
+// for the mixed V-cycle.  Storage is np x np (np = 64 nblk), the permuted
+// DOFs plus identity padding.
 struct CoarseFactor {
-  int n = 0, bw = 0, nblk = 0, kband = 0, nring = 0;
-  double *A = nullptr;     // n x n matrix (lower triangle read; overwritten)
-  double *L = nullptr;     // n x n Cholesky factor
+  int n = 0, bw = 0, np = 0, nblk = 0;
+  double *A = nullptr;     // np x np matrix in storage order (overwritten)
+  double *L = nullptr;     // np x np Cholesky factor
   double *Dinv = nullptr;  // nblk 64x64 inverses of L's diagonal blocks
   double *W = nullptr, *WT = nullptr;
   float *W32 = nullptr, *WT32 = nullptr;
   int *info = nullptr;
-  double *cw64 = nullptr;  // 2n work (gathered input, intermediate)
+  double *cw64 = nullptr;  // 2 np work (gathered input, intermediate)
   float *cw32 = nullptr;
-  bool allocate(int n, int bw, bool f32);
+  int *pos = nullptr;      // logical DOF -> storage index (device, n)
+  int *ipos = nullptr;     // storage index -> logical DOF or -1 (device, np)
+  // schedule: launch q runs tasks [launch_first[q], +launch_count[q]);
+  // W rows per tile column (colinfo: offset, count into rows; rows: I,
+  // offset, count into klist)
+  CholTask *dtasks = nullptr;
+  int ntask = 0;
+  std::vector<int> launch_first, launch_count;
+  int2 *dcolinfo = nullptr;
+  int4 *drows = nullptr;
+  int *dklist = nullptr;
+  // planes > 1 with n divisible by planes: nested dissection over the planes
+  // (n / planes consecutive DOFs each, coupled to adjacent planes only);
+  // otherwise banded with half bandwidth bw
+  bool allocate(int n, int bw, bool f32, int planes);
   void release();
+  void begin_assembly();  // A = 0 with identity padding
   int factor();  // TOP3D_OK or TOP3D_ERR_NOT_SPD / TOP3D_ERR_HIP
   template <typename T> const T *w() const {
     if constexpr (sizeof(T) == 8) return (const T *)W; else return (const T *)W32;
