@@ -676,9 +676,9 @@ struct TileShape {
 // override: TOP3D_TILE_L<level>=TY,TZ,NX (NX = 0: from the occupancy).
 //
 // Shapes measured on MI355X at 256x128x128 (fine) / 128x64x64 (level 1):
-// small workgroups (about 200-260 threads) for the fine level, so that several
-// share a CU and their barriers interleave; wide, short x-marches for level
-// 1.  The x-march length follows from the number of workgroups the device
+// for the FP32 fine level 65 x 7 (455 threads: 7 of 8 waves full, 84% of the
+// columns own nodes) ran 49 us in the V-cycle against 55 us for 33 x 8 (264
+// threads, a fifth wave with 8 lanes); wide, short x-marches for level 1.  The x-march length follows from the number of workgroups the device
 // holds at once (`slots`): a launch that overflows them by a few
 // workgroups pays a whole extra round, so NX is chosen to minimise
 // rounds x (NX + 3), the per-workgroup cost in slab steps.
@@ -692,7 +692,7 @@ static TileShape tile_shape(const Level &L, int lvl, int elem_bytes, int &nx_fix
   struct Rule {
     int ycap, tz, nx;
   };
-  static constexpr Rule kRules[2][2] = {{{17, 12, 0}, {33, 8, 0}},  // level 0: f64, f32
+  static constexpr Rule kRules[2][2] = {{{17, 12, 0}, {65, 7, 0}},  // level 0: f64, f32
                                         {{65, 6, 8}, {65, 12, 4}}};  // level 1
   const Rule r = kRules[lvl == 0 ? 0 : 1][elem_bytes == 4 ? 1 : 0];
   s.TY = std::max(2, std::min(L.nely + 1, r.ycap));
@@ -871,16 +871,16 @@ void launch_level1_tile(const DevGrid &g, int mode, const T *E, const T *in, T *
 // formed from the moduli as in the kJacobiDiag sweep:
 //   out = w b / (KE[c][c] sum of the node's 8 element moduli)  (real nodes),
 //   out = w b at fixed DOFs, 0 on every other node.
+// One thread per node of a node plane (blockIdx.y = plane).
 template <typename TB, typename T>
 __global__ __launch_bounds__(256) void k_jacobi_zero_fine(
     int nx, int ny, int nz, int wy, int wz, int few, int fez, const TB *__restrict__ b,
     const T *__restrict__ E, const uint8_t *__restrict__ fix, Blocks<T> B, T omega,
     T *__restrict__ out) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int k = blockIdx.y * blockDim.y + threadIdx.y;
-  const int i = blockIdx.z;
-  if (j >= wy || k >= wz) return;
-  const int64_t node = ((int64_t)i * wz + k) * wy + j;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= wy * wz) return;
+  const int j = t % wy, k = t / wy, i = blockIdx.y;
+  const int64_t node = (int64_t)i * wz * wy + t;
   T *po = out + 3 * node;
   if (!(i >= 1 && i <= nx && j >= 1 && j <= ny && k >= 1 && k <= nz)) {
     po[0] = po[1] = po[2] = T(0);
@@ -907,7 +907,7 @@ bool launch_fine_jacobi_zero(const DevGrid &g, const TB *b, const T *E, T omega,
   const Level &L = g.lv[0];
   Blocks<T> B;
   if constexpr (sizeof(T) == 8) B = g_coef.b64; else B = g_coef.b32;
-  dim3 bs(64, 4, 1), gs((L.wy + 63) / 64, (L.wz + 3) / 4, L.wx);
+  dim3 bs(256), gs((L.wy * L.wz + 255) / 256, L.wx);
   hipLaunchKernelGGL((k_jacobi_zero_fine<TB, T>), gs, bs, 0, stream(), L.nx, L.ny, L.nz, L.wy,
                      L.wz, g.ewy, g.ewz, b, E, g.fixmask[0], B, omega, out);
   return true;

@@ -343,6 +343,55 @@ __global__ __launch_bounds__(256) void k_stencil(
   }
 }
 
+// Solver variant: one thread per (interior node, row r) with blockIdx.y = r,
+// the same arithmetic per row as k_stencil; nodes outside the interior are
+// not written (the solver's vectors of these levels stay zero there).
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_stencil_rows(
+    int nx, int ny, int nz, int wy, int wz, const T *__restrict__ S,
+    const T *__restrict__ in, T *__restrict__ out, const T *__restrict__ b,
+    const T *__restrict__ invD, T omega) {
+  const int64_t nint = (int64_t)nx * ny * nz;
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  if (n >= nint) return;
+  const int j = (int)(n % ny), k = (int)((n / ny) % nz), i = (int)(n / ((int64_t)ny * nz));
+  const int64_t node = ((int64_t)(i + 1) * wz + (k + 1)) * wy + (j + 1);
+  T acc = 0;
+#pragma unroll
+  for (int dx = -1; dx <= 1; dx++)
+#pragma unroll
+    for (int dz = -1; dz <= 1; dz++)
+#pragma unroll
+      for (int dy = -1; dy <= 1; dy++) {
+        const int nbi = (dx + 1) * 9 + (dz + 1) * 3 + (dy + 1);
+        const int64_t q = node + ((int64_t)dx * wz + dz) * wy + dy;
+        const T *s = S + (int64_t)(nbi * 9 + 3 * r) * nint + n;
+        acc += s[0] * in[3 * q] + s[nint] * in[3 * q + 1] + s[2 * nint] * in[3 * q + 2];
+      }
+  const int64_t o = 3 * node + r;
+  const T bb = MODE == kApply ? T(0) : b[o];
+  const T dd = MODE == kJacobi ? invD[o] : T(0);
+  out[o] = finish<T>(MODE, acc, in[o], bb, dd, omega, false);
+}
+
+template <typename T>
+void launch_stencil_rows(const DevGrid &g, int l, int mode, const T *S, const T *in, T *out,
+                         const T *b, const T *invD, T omega) {
+  const Level &L = g.lv[l];
+  const int64_t nint = L.interior_nodes();
+  dim3 bs(256), gs((unsigned)((nint + 255) / 256), 3);
+  if (mode == kApply)
+    hipLaunchKernelGGL((k_stencil_rows<T, kApply>), gs, bs, 0, stream(), L.nx, L.ny, L.nz,
+                       L.wy, L.wz, S, in, out, b, invD, omega);
+  else if (mode == kJacobi)
+    hipLaunchKernelGGL((k_stencil_rows<T, kJacobi>), gs, bs, 0, stream(), L.nx, L.ny, L.nz,
+                       L.wy, L.wz, S, in, out, b, invD, omega);
+  else
+    hipLaunchKernelGGL((k_stencil_rows<T, kResidual>), gs, bs, 0, stream(), L.nx, L.ny,
+                       L.nz, L.wy, L.wz, S, in, out, b, invD, omega);
+}
+
 template <typename T>
 void launch_stencil(const DevGrid &g, int l, int mode, const T *S, const T *in,
                     T *out, const T *b, const T *invD, T omega) {
@@ -594,6 +643,8 @@ void launch_invdiag_stencil(const DevGrid &g, int l, const double *S, T *invD) {
                                  const T *, const T *, T, int);                          \
   template void launch_stencil<T>(const DevGrid &, int, int, const T *, const T *, T *,  \
                                   const T *, const T *, T);                              \
+  template void launch_stencil_rows<T>(const DevGrid &, int, int, const T *, const T *,  \
+                                       T *, const T *, const T *, T);                    \
   template void launch_restrict<T>(const DevGrid &, int, const T *, T *);                \
   template void launch_prolong_add<T>(const DevGrid &, int, const T *, T *);             \
   template void launch_prolong<T>(const DevGrid &, int, const T *, T *);                 \

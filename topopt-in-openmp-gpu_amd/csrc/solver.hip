@@ -160,7 +160,7 @@ void SolverState::op(int l, int mode, const T *in, T *out, const T *b) {
   else if (l == 1)
     launch_level1<T>(*g, mode, young_t<T>(), in, out, b, invD, omega, bcsym);
   else
-    launch_stencil<T>(*g, l, mode, stencil_t<T>(l), in, out, b, invD, omega);
+    launch_stencil_rows<T>(*g, l, mode, stencil_t<T>(l), in, out, b, invD, omega);
 }
 
 // nswp damped-Jacobi sweeps on level l starting from z = 0 (first sweep

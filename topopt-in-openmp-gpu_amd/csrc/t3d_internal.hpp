@@ -115,6 +115,11 @@ void launch_level1(const DevGrid &g, int mode, const T *E, const T *in, T *out,
 template <typename T>
 void launch_stencil(const DevGrid &g, int l, int mode, const T *S, const T *in,
                     T *out, const T *b, const T *invD, T omega);
+// the same on the interior nodes only (one thread per node and row); the
+// other nodes of `out` are left as they are
+template <typename T>
+void launch_stencil_rows(const DevGrid &g, int l, int mode, const T *S, const T *in,
+                         T *out, const T *b, const T *invD, T omega);
 // fine-level sweep from z = 0 with the diagonal formed from E; false if the
 // element matrix does not allow it
 template <typename TB, typename T>
