@@ -282,7 +282,9 @@ int top3d_synchronize(void);
 struct Top3dSolveStats {
   int iterations;        /* CG iterations run (finalIter + 1) */
   double relres;         /* final relative residual (FP64) */
-  double setup_ms;       /* per-solve set-up: levels, diagonals, factor */
+  double setup_ms;       /* host time to queue the per-solve set-up (the
+                            coarse levels and factor run on a second stream,
+                            overlapping the first V-cycle) */
   double solve_ms;       /* CG loop */
   int status;            /* TOP3D_OK or an error code */
 };

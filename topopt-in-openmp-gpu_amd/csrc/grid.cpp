@@ -37,7 +37,21 @@ bool hip_ok(hipError_t e, const char *what) {
   return false;
 }
 
+// stream of the enclosing StreamScope on this thread, if any
+thread_local hipStream_t t_scope_stream = nullptr;
+thread_local bool t_scope_on = false;
+
+StreamScope::StreamScope(hipStream_t s) : prev(t_scope_stream), prev_on(t_scope_on) {
+  t_scope_stream = s;
+  t_scope_on = true;
+}
+StreamScope::~StreamScope() {
+  t_scope_stream = prev;
+  t_scope_on = prev_on;
+}
+
 hipStream_t stream() {
+  if (t_scope_on) return t_scope_stream;
   if (!g_stream_set) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_stream_set) {

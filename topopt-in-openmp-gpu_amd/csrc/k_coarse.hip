@@ -37,59 +37,75 @@ __device__ __forceinline__ double qweight(int cx, int cy, int cz, int b, int B) 
 
 // One workgroup (576 threads, one per matrix entry) per parent element.
 // SRC = 0: children are level-1 elements built from fine moduli and KE_1;
-// SRC = 1: children are stored level-(l-1) element matrices.
+// SRC = 1: children are stored level-(l-1) element matrices.  The 8 children
+// are staged in LDS together and the two products run for all of them
+// between two barriers.
 template <int SRC>
 __global__ __launch_bounds__(576) void k_galerkin(
     int nelx, int nely, int nelz,        // parent level elements
     int cnely, int cnelz,                // child level elements (SRC = 1)
     int few, int fez, const double *__restrict__ KE1, const double *__restrict__ E,
     const double *__restrict__ Kchild, double *__restrict__ Kparent) {
-  __shared__ double Kc[576];
-  __shared__ double Tm[576];
+  __shared__ double Kall[8 * 576];  // child matrices
+  __shared__ double Tm[8 * 576];    // Kc Q_c per child
+  __shared__ double Wq[8 * 64];     // Q_c: weight of parent node B at child node b, [c][b][B]
   const int t = threadIdx.x;
   const int r = t / 24, s = t % 24;
   const int64_t pe = blockIdx.x;  // compact parent index ((I*nelz + K)*nely + J), 0-based
   const int J = (int)(pe % nely);
   const int K = (int)((pe / nely) % nelz);
   const int I = (int)(pe / ((int64_t)nely * nelz));
+  {
+    double kv[8];
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      const int ci = 2 * I + cn_x(c), cj = 2 * J + cn_y(c), ck = 2 * K + cn_z(c);
+      if (SRC == 0) {
+        double v = 0.0;
+        for (int ii = 0; ii < 2; ii++)
+          for (int kk = 0; kk < 2; kk++)
+            for (int jj = 0; jj < 2; jj++) {
+              const int64_t fe = ((int64_t)(2 * ci + ii + 1) * fez + (2 * ck + kk + 1)) * few +
+                                 (2 * cj + jj + 1);
+              v += E[fe] * KE1[576 * (4 * ii + 2 * kk + jj) + t];
+            }
+        kv[c] = v;
+      } else {
+        kv[c] = Kchild[576 * (((int64_t)ci * cnelz + ck) * cnely + cj) + t];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 8; c++) Kall[576 * c + t] = kv[c];
+  }
+  if (t < 512) {
+    const int c = t >> 6, q = t & 63;
+    Wq[t] = qweight(cn_x(c), cn_y(c), cn_z(c), q >> 3, q & 7);
+  }
+  __syncthreads();
+  // Tm_c[a][S] = sum_b Kc[a][3b + cs] Q_c[b][Snode]
+  {
+    const int a = r, Sn = s / 3, cs = s % 3;
+#pragma unroll 1
+    for (int c = 0; c < 8; c++) {
+      double v = 0.0;
+#pragma unroll
+      for (int b = 0; b < 8; b++)
+        v += Kall[576 * c + 24 * a + 3 * b + cs] * Wq[64 * c + 8 * b + Sn];
+      Tm[576 * c + t] = v;
+    }
+  }
+  __syncthreads();
+  // P[R][S] = sum_c sum_a Q_c[a][Rnode] Tm_c[3a + cr][S]
   double acc = 0.0;
-  for (int c = 0; c < 8; c++) {
-    const int cx = cn_x(c), cy = cn_y(c), cz = cn_z(c);  // child octant
-    const int ci = 2 * I + cx, cj = 2 * J + cy, ck = 2 * K + cz;  // child, 0-based
-    double kv;
-    if (SRC == 0) {
-      kv = 0.0;
-      for (int ii = 0; ii < 2; ii++)
-        for (int kk = 0; kk < 2; kk++)
-          for (int jj = 0; jj < 2; jj++) {
-            const int64_t fe = ((int64_t)(2 * ci + ii + 1) * fez + (2 * ck + kk + 1)) * few +
-                               (2 * cj + jj + 1);
-            kv += E[fe] * KE1[576 * (4 * ii + 2 * kk + jj) + t];
-          }
-    } else {
-      const int64_t ce = ((int64_t)ci * cnelz + ck) * cnely + cj;
-      kv = Kchild[576 * ce + t];
-    }
-    Kc[t] = kv;
-    __syncthreads();
-    // Tm[a][S] = sum_b Kc[a][3b + cs] Q[b][Snode]
-    {
-      const int a = r, S = s, Sn = S / 3, cs = S % 3;
+  {
+    const int Rn = r / 3, cr = r % 3;
+#pragma unroll 1
+    for (int c = 0; c < 8; c++) {
       double v = 0.0;
 #pragma unroll
-      for (int b = 0; b < 8; b++) v += Kc[24 * a + 3 * b + cs] * qweight(cx, cy, cz, b, Sn);
-      Tm[t] = v;
-    }
-    __syncthreads();
-    // P[R][S] += sum_a Q[a][Rnode] Tm[3a + cr][S]
-    {
-      const int Rn = r / 3, cr = r % 3;
-      double v = 0.0;
-#pragma unroll
-      for (int a = 0; a < 8; a++) v += qweight(cx, cy, cz, a, Rn) * Tm[24 * (3 * a + cr) + s];
+      for (int a = 0; a < 8; a++) v += Wq[64 * c + 8 * a + Rn] * Tm[576 * c + 24 * (3 * a + cr) + s];
       acc += v;
     }
-    __syncthreads();
   }
   Kparent[576 * pe + t] = acc;
 }
@@ -164,56 +180,78 @@ void launch_elem_galerkin(const DevGrid &g, int l, const double *Kchild, double 
                      Kparent);
 }
 
-// 27-point stencil from element matrices, one thread per (node, neighbour).
+// 27-point stencil from element matrices, one thread per (node, neighbour):
+// threadIdx.x = neighbour nb, kNsNodes consecutive nodes per workgroup, so
+// that the threads of a node read the same three rows of each of its
+// element matrices; the workgroup's 243 x kNsNodes values are staged in LDS
+// and written as kNsNodes-long runs of each S plane.
 // S[(nb*9 + r*3 + c) * nint + n]; symmetric elimination of fixed DOFs.
 // bc_symmetric = 0 keeps the columns of fixed DOFs and only replaces fixed
 // rows by identity rows (the matrix-free convention, stencil_methods.c:515-518).
-__global__ void k_node_stencil(int nelx, int nely, int nelz, int wy, int wz,
-                               const double *__restrict__ Ke,
-                               const uint8_t *__restrict__ fix, double *__restrict__ S,
-                               int bc_symmetric) {
+// S32: an FP32 copy written alongside when given.
+constexpr int kNsNodes = 32;
+
+__global__ __launch_bounds__(27 * kNsNodes) void k_node_stencil(
+    int nelx, int nely, int nelz, int wy, int wz, const double *__restrict__ Ke,
+    const uint8_t *__restrict__ fix, double *__restrict__ S, float *__restrict__ S32,
+    int bc_symmetric) {
+  __shared__ double tile[243 * kNsNodes];
   const int64_t nx = nelx + 1, ny = nely + 1, nz = nelz + 1, nint = nx * ny * nz;
-  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= 27 * nint) return;
-  const int nb = (int)(tid / nint);
-  const int64_t n = tid % nint;
-  const int j = (int)(n % ny), k = (int)((n / ny) % nz), i = (int)(n / (ny * nz));
-  const int dx = nb / 9 - 1, dz = (nb / 3) % 3 - 1, dy = nb % 3 - 1;
-  const int i2 = i + dx, j2 = j + dy, k2 = k + dz;
-  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  const bool nb_in = i2 >= 0 && i2 < nx && j2 >= 0 && j2 < ny && k2 >= 0 && k2 < nz;
-  if (nb_in) {
-    for (int e = 0; e < 8; e++) {  // elements (0-based) containing node n
-      const int ex = (e & 1) ? 0 : -1, ey = (e & 2) ? 0 : -1, ez = (e & 4) ? 0 : -1;
-      const int ie = i + ex, je = j + ey, ke = k + ez;
-      if (ie < 0 || ie >= nelx || je < 0 || je >= nely || ke < 0 || ke >= nelz) continue;
-      const int ox = dx - ex, oy = dy - ey, oz = dz - ez;  // neighbour offset in element
-      if (ox < 0 || ox > 1 || oy < 0 || oy > 1 || oz < 0 || oz > 1) continue;
-      const int a = cn_of(-ex, -ey, -ez), b = cn_of(ox, oy, oz);
-      const double *K = Ke + 576 * (((int64_t)ie * nelz + ke) * nely + je);
+  const int nb = threadIdx.x;
+  const int64_t n0 = (int64_t)blockIdx.x * kNsNodes;
+  const int64_t n = n0 + threadIdx.y;
+  const int nloc = (int)min((int64_t)kNsNodes, nint - n0);
+  if (n < nint) {
+    const int j = (int)(n % ny), k = (int)((n / ny) % nz), i = (int)(n / (ny * nz));
+    const int dx = nb / 9 - 1, dz = (nb / 3) % 3 - 1, dy = nb % 3 - 1;
+    const int i2 = i + dx, j2 = j + dy, k2 = k + dz;
+    double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const bool nb_in = i2 >= 0 && i2 < nx && j2 >= 0 && j2 < ny && k2 >= 0 && k2 < nz;
+    if (nb_in) {
+      for (int e = 0; e < 8; e++) {  // elements (0-based) containing node n
+        const int ex = (e & 1) ? 0 : -1, ey = (e & 2) ? 0 : -1, ez = (e & 4) ? 0 : -1;
+        const int ie = i + ex, je = j + ey, ke = k + ez;
+        if (ie < 0 || ie >= nelx || je < 0 || je >= nely || ke < 0 || ke >= nelz) continue;
+        const int ox = dx - ex, oy = dy - ey, oz = dz - ez;  // neighbour offset in element
+        if (ox < 0 || ox > 1 || oy < 0 || oy > 1 || oz < 0 || oz > 1) continue;
+        const int a = cn_of(-ex, -ey, -ez), b = cn_of(ox, oy, oz);
+        const double *K = Ke + 576 * (((int64_t)ie * nelz + ke) * nely + je);
 #pragma unroll
-      for (int r = 0; r < 3; r++)
+        for (int r = 0; r < 3; r++)
 #pragma unroll
-        for (int c = 0; c < 3; c++) v[3 * r + c] += K[24 * (3 * a + r) + 3 * b + c];
-    }
-    const uint8_t m1 = fix[((int64_t)(i + 1) * wz + (k + 1)) * wy + (j + 1)];
-    const uint8_t m2 = fix[((int64_t)(i2 + 1) * wz + (k2 + 1)) * wy + (j2 + 1)];
-    for (int r = 0; r < 3; r++)
-      for (int c = 0; c < 3; c++) {
-        const bool fr = (m1 >> r) & 1, fc = bc_symmetric && ((m2 >> c) & 1);
-        if (fr || fc) v[3 * r + c] = (nb == 13 && r == c) ? 1.0 : 0.0;
+          for (int c = 0; c < 3; c++) v[3 * r + c] += K[24 * (3 * a + r) + 3 * b + c];
       }
+      const uint8_t m1 = fix[((int64_t)(i + 1) * wz + (k + 1)) * wy + (j + 1)];
+      const uint8_t m2 = fix[((int64_t)(i2 + 1) * wz + (k2 + 1)) * wy + (j2 + 1)];
+      for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+          const bool fr = (m1 >> r) & 1, fc = bc_symmetric && ((m2 >> c) & 1);
+          if (fr || fc) v[3 * r + c] = (nb == 13 && r == c) ? 1.0 : 0.0;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 9; q++) tile[(nb * 9 + q) * kNsNodes + threadIdx.y] = v[q];
   }
-  for (int q = 0; q < 9; q++) S[(int64_t)(nb * 9 + q) * nint + n] = v[q];
+  __syncthreads();
+  const int tid = threadIdx.y * 27 + threadIdx.x;
+  for (int e = tid; e < 243 * kNsNodes; e += 27 * kNsNodes) {
+    const int q = e / kNsNodes, nn = e % kNsNodes;
+    if (nn < nloc) {
+      const double val = tile[e];
+      S[(int64_t)q * nint + n0 + nn] = val;
+      if (S32) S32[(int64_t)q * nint + n0 + nn] = (float)val;
+    }
+  }
 }
 
 void launch_node_stencil(const DevGrid &g, int l, const double *Ke, double *S,
-                         int bc_symmetric) {
+                         int bc_symmetric, float *S32) {
   const Level &L = g.lv[l];
-  const int64_t nthreads = 27 * L.interior_nodes();
-  const int bs = 256;
-  hipLaunchKernelGGL(k_node_stencil, dim3((nthreads + bs - 1) / bs), dim3(bs), 0, stream(),
-                     L.nelx, L.nely, L.nelz, L.wy, L.wz, Ke, g.fixmask[l], S, bc_symmetric);
+  const int64_t nint = L.interior_nodes();
+  hipLaunchKernelGGL(k_node_stencil, dim3((unsigned)((nint + kNsNodes - 1) / kNsNodes)),
+                     dim3(27, kNsNodes), 0, stream(),
+                     L.nelx, L.nely, L.nelz, L.wy, L.wz, Ke, g.fixmask[l], S, S32,
+                     bc_symmetric);
 }
 
 // the assembled coarsest level into the coarse factor's np x np storage

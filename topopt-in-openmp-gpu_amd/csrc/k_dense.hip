@@ -336,14 +336,17 @@ __global__ __launch_bounds__(256) void k_chol_task(double *__restrict__ A, doubl
 // elimination schedule).  W_K is read back from W (written by this
 // workgroup before a barrier); the L_IK tiles are staged through LDS with the
 // next tile's loads in flight during the current product; the rows of
-// L_II^{-1} a wave needs are loaded at the start of the row.
+// L_II^{-1} a wave needs are loaded at the start of the row.  W is always
+// written; WT, W32 and WT32 (FP32 copies) when given.
 __global__ __launch_bounds__(256) void k_linv_seq(const double *__restrict__ Lf,
                                                   const double *__restrict__ Dinv, int np,
                                                   const int2 *__restrict__ colinfo,
                                                   const int4 *__restrict__ rows,
                                                   const int *__restrict__ klist,
                                                   double *__restrict__ W,
-                                                  double *__restrict__ WT) {
+                                                  double *__restrict__ WT,
+                                                  float *__restrict__ W32,
+                                                  float *__restrict__ WT32) {
   __shared__ double Bx[kNB * 17];
   __shared__ double Lt[2 * kNB * kLd];
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
@@ -356,7 +359,11 @@ __global__ __launch_bounds__(256) void k_linv_seq(const double *__restrict__ Lf,
     const double v = Dinv[(int64_t)J * kNB * kNB + r * kNB + cofs + j];
     const int64_t gr = (int64_t)J * kNB + r;
     W[gr * np + c0 + j] = v;
-    WT[(int64_t)(c0 + j) * np + gr] = v;
+    if (WT) WT[(int64_t)(c0 + j) * np + gr] = v;
+    if (W32) {
+      W32[gr * np + c0 + j] = (float)v;
+      WT32[(int64_t)(c0 + j) * np + gr] = (float)v;
+    }
   }
   auto load_tile = [&](int I, int K, double (&reg)[16]) {
 #pragma unroll
@@ -412,7 +419,11 @@ __global__ __launch_bounds__(256) void k_linv_seq(const double *__restrict__ Lf,
       const int64_t gr = (int64_t)I * kNB + 16 * w + lk + 4 * v;
       const double val = -o[v];
       W[gr * np + gc] = val;
-      WT[(int64_t)gc * np + gr] = val;
+      if (WT) WT[(int64_t)gc * np + gr] = val;
+      if (W32) {
+        W32[gr * np + gc] = (float)val;
+        WT32[(int64_t)gc * np + gr] = (float)val;
+      }
     }
     __syncthreads();
   }
@@ -544,19 +555,14 @@ void CoarseFactor::begin_assembly() {
                      A);
 }
 
-int CoarseFactor::factor() {
-  hipStream_t s = stream();
-  hipMemsetAsync(info, 0, sizeof(int), s);
-  for (size_t q = 0; q < launch_first.size(); q++)
-    hipLaunchKernelGGL(k_chol_task, dim3(launch_count[q]), dim3(256), 0, s, A, L, Dinv, np,
-                       dtasks + launch_first[q], info);
-  hipLaunchKernelGGL(k_linv_seq, dim3(np / kStrip), dim3(256), 0, s, L, Dinv, np, dcolinfo,
-                     drows, dklist, W, WT);
-  if (W32) {
-    launch_convert<float, double>((int64_t)np * np, W, W32);
-    launch_convert<float, double>((int64_t)np * np, WT, WT32);
-  }
+int CoarseFactor::factor(bool f32out) {
+  factor_async(f32out);
+  return check_info();
+}
+
+int CoarseFactor::check_info() {
   int hinfo = 0;
+  hipStream_t s = stream();
   hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s);
   if (!T3D_HIP(hipStreamSynchronize(s))) return TOP3D_ERR_HIP;
   if (hinfo) {
@@ -564,6 +570,18 @@ int CoarseFactor::factor() {
     return TOP3D_ERR_NOT_SPD;
   }
   return TOP3D_OK;
+}
+
+void CoarseFactor::factor_async(bool f32out) {
+  hipStream_t s = stream();
+  hipMemsetAsync(info, 0, sizeof(int), s);
+  for (size_t q = 0; q < launch_first.size(); q++)
+    hipLaunchKernelGGL(k_chol_task, dim3(launch_count[q]), dim3(256), 0, s, A, L, Dinv, np,
+                       dtasks + launch_first[q], info);
+  const bool f32 = f32out && W32;  // FP32 W, W^T for the mixed V-cycle, else FP64 W^T
+  hipLaunchKernelGGL(k_linv_seq, dim3(np / kStrip), dim3(256), 0, s, L, Dinv, np, dcolinfo,
+                     drows, dklist, W, f32 ? nullptr : WT, f32 ? W32 : nullptr,
+                     f32 ? WT32 : nullptr);
 }
 
 // ---------------------------------------------------------------------------

@@ -79,11 +79,17 @@ bool SolverState::allocate(DevGrid *grid, int nlev) {
   if (!T3D_HIP(hipHostMalloc((void **)&hscal, sizeof(double) * 16, hipHostMallocDefault)))
     return false;
   if (!T3D_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming))) return false;
+  if (!T3D_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking)) ||
+      !T3D_HIP(hipEventCreateWithFlags(&ev_base, hipEventDisableTiming)) ||
+      !T3D_HIP(hipEventCreateWithFlags(&ev_stencil, hipEventDisableTiming)) ||
+      !T3D_HIP(hipEventCreateWithFlags(&ev_coarse, hipEventDisableTiming)))
+    return false;
   return T3D_HIP(hipStreamSynchronize(stream())) && r && p && q && z;
 }
 
 void SolverState::release() {
   hipStreamSynchronize(stream());
+  if (aux) hipStreamSynchronize(aux);
   auto f = [](void *ptr) { if (ptr) hipFree(ptr); };
   for (int l = 0; l < nl; l++) {
     f(invD64[l]);
@@ -103,37 +109,70 @@ void SolverState::release() {
   f(scal); f(partials); f(cgpart);
   if (hscal) hipHostFree(hscal);
   if (ev) hipEventDestroy(ev);
+  for (hipEvent_t e : {ev_base, ev_stencil, ev_coarse})
+    if (e) hipEventDestroy(e);
+  if (aux) hipStreamDestroy(aux);
+  aux = nullptr;
   for (hipEvent_t e : sw_ev) hipEventDestroy(e);
   sw_ev.clear();
   magic = 0;
 }
 
 // ---------------------------------------------------------------------------
-// per-solve set-up (stencil_solvers.c:283-302)
-bool SolverState::setup(const double *x) {
+// per-solve set-up (stencil_solvers.c:283-302): the fine levels on the
+// library stream
+bool SolverState::setup_main(const double *x) {
   launch_young<double>(*g, x, E64);
   if (mixed) launch_convert<float, double>(g->nelem, E64, E32);
   // the fine Jacobi sweeps form their diagonal from E when they can
   const bool fdiag = fine_sweep_forms_diagonal();
   if (!fdiag) launch_invdiag_mf<double>(*g, 0, E64, invD64[0]);
   launch_invdiag_mf<double>(*g, 1, E64, invD64[1]);
-  for (int l = 2; l < nl; l++) {
-    if (l == 2)
-      launch_elem_level2(*g, E64, Ke[2]);
-    else
-      launch_elem_galerkin(*g, l, Ke[l - 1], Ke[l]);
-    launch_node_stencil(*g, l, Ke[l], S64[l], 1);
-    launch_invdiag_stencil<double>(*g, l, S64[l], invD64[l]);
-  }
-  launch_stencil_to_dense(*g, nl - 1, S64[nl - 1], coarse);
-  if (coarse.factor() != TOP3D_OK) return false;
-  if (mixed) {
-    for (int l = fdiag ? 1 : 0; l < nl; l++)
+  if (mixed)
+    for (int l = fdiag ? 1 : 0; l < 2; l++)
       launch_convert<float, double>(g->lv[l].ndof, invD64[l], invD32[l]);
-    for (int l = 2; l < nl - 1; l++)
-      launch_convert<float, double>(243 * g->lv[l].interior_nodes(), S64[l], S32[l]);
+  return T3D_HIP(hipEventRecord(ev_base, stream()));
+}
+
+// The assembled levels and the coarse factorization, queued on `aux` after
+// the set-up on the library stream (ev_base).
+bool SolverState::setup_aux() {
+  {
+    StreamScope on_aux(aux);
+    if (!T3D_HIP(hipStreamWaitEvent(aux, ev_base, 0))) return false;
+    for (int l = 2; l < nl; l++) {
+      if (l == 2)
+        launch_elem_level2(*g, E64, Ke[2]);
+      else
+        launch_elem_galerkin(*g, l, Ke[l - 1], Ke[l]);
+      // the smoother levels of the mixed V-cycle also get the FP32 stencil
+      const bool smoother = l < nl - 1;
+      launch_node_stencil(*g, l, Ke[l], S64[l], 1, (mixed && smoother) ? S32[l] : nullptr);
+      launch_invdiag_stencil<double>(*g, l, S64[l], invD64[l]);
+      if (mixed && smoother) launch_convert<float, double>(g->lv[l].ndof, invD64[l], invD32[l]);
+      if (l == nl - 2) hipEventRecord(ev_stencil, aux);
+    }
+    if (nl < 4) hipEventRecord(ev_stencil, aux);  // no assembled smoother level
+    launch_stencil_to_dense(*g, nl - 1, S64[nl - 1], coarse);
+    coarse.factor_async(mixed);
+    if (!T3D_HIP(hipEventRecord(ev_coarse, aux))) return false;
   }
+  wait_stencil = wait_coarse = true;
   return true;
+}
+
+// the library stream waits for the assembled smoother levels / the coarse
+// factor built on `aux` (once per solve)
+void SolverState::join_stencil() {
+  if (!wait_stencil) return;
+  hipStreamWaitEvent(stream(), ev_stencil, 0);
+  wait_stencil = false;
+}
+void SolverState::join_coarse() {
+  if (!wait_coarse) return;
+  join_stencil();
+  hipStreamWaitEvent(stream(), ev_coarse, 0);
+  wait_coarse = false;
 }
 
 // ---------------------------------------------------------------------------
@@ -195,11 +234,13 @@ T *SolverState::smooth(int l, const T *b, T *cur, T *oth) {
 template <typename T>
 void SolverState::vcycle(int l) {
   T *r_ = vec<T>(l, kR), *z_ = vec<T>(l, kZ), *d_ = vec<T>(l, kD), *t_ = tmp<T>(l);
+  if (l >= 2) join_stencil();
   T *zc = smooth_from_zero<T>(l, r_, z_, t_);
   T *other = (zc == z_) ? t_ : z_;
   op<T>(l, 2, zc, d_, r_);  // d = r - K z
   launch_restrict<T>(*g, l, d_, vec<T>(l + 1, kR));
   if (nl == l + 2) {
+    join_coarse();
     launch_coarse_apply<T>(*g, l + 1, coarse, vec<T>(l + 1, kR), vec<T>(l + 1, kZ));
   } else {
     vcycle<T>(l + 1);
@@ -241,18 +282,25 @@ int SolverState::solve(const double *x, const double *b, double *u, double tol, 
   stats = Top3dSolveStats{};
   const int64_t n0 = g->lv[0].ndof;
   hipMemcpyAsync(z, u, sizeof(double) * n0, hipMemcpyDeviceToDevice, stream());
-  if (!setup(x)) {
+  if (!setup_main(x)) {
     stats.status = top3d_last_error();
     return stats.status;
   }
   launch_fine<double>(*g, 2, E64, z, r, b, nullptr, 0.0, bcsym);  // r = b - K z
   launch_dot<double>(b, b, n0, partials, scal + 4);
   if (mixed) launch_convert<float, double>(n0, r, v32[0][kR]);
+  hscal[4] = 0.0;
   hipMemcpyAsync(hscal + 4, scal + 4, sizeof(double), hipMemcpyDeviceToHost, stream());
-  if (!T3D_HIP(hipStreamSynchronize(stream()))) return TOP3D_ERR_HIP;
+  // the coarse set-up is queued while the library stream computes r
+  if (!setup_aux()) {
+    join_coarse();
+    hipStreamSynchronize(stream());
+    stats.status = top3d_last_error();
+    return stats.status;
+  }
   const auto t1 = clk::now();
   stats.setup_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-  const double bnorm = std::sqrt(hscal[4]);
+  double bnorm = 0.0;  // read once the first iteration's event has completed
   int status = TOP3D_OK;
   int iter = 0;
   double relres = 0.0;
@@ -290,6 +338,7 @@ int SolverState::solve(const double *x, const double *b, double *u, double tol, 
       status = TOP3D_ERR_HIP;
       break;
     }
+    if (iter == 0) bnorm = std::sqrt(hscal[4]);
     relres = std::sqrt(hscal[3]) / bnorm;
     if (final_iter) *final_iter = iter;
     if (final_res) *final_res = (float)relres;
@@ -300,7 +349,12 @@ int SolverState::solve(const double *x, const double *b, double *u, double tol, 
     }
     if (relres < tol) break;
   }
+  join_coarse();  // the factorization has finished before the solve returns
   if (!T3D_HIP(hipStreamSynchronize(stream()))) status = TOP3D_ERR_HIP;
+  if (status == TOP3D_OK) {
+    const int rc = coarse.check_info();
+    if (rc != TOP3D_OK) status = rc;
+  }
   if (status == TOP3D_OK && iter >= max_iter) {
     status = TOP3D_ERR_NO_CONVERGE;
     set_error(status, "CG: relres %.3e above tol after %d iterations", relres, max_iter);

@@ -39,6 +39,15 @@ struct SolverState {
   double *scal = nullptr, *partials = nullptr, *hscal = nullptr;
   double *cgpart = nullptr;  // partial sums of r.z, p.q, r.r
   hipEvent_t ev = nullptr;
+  // The assembled levels and the coarse factorization are built on `aux`,
+  // concurrently with the start of the solve on the library stream, which
+  // waits for ev_stencil before its first sweep on level 2 and for ev_coarse
+  // before its first coarse solve.
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_base = nullptr, ev_stencil = nullptr, ev_coarse = nullptr;
+  bool wait_stencil = false, wait_coarse = false;
+  void join_stencil();
+  void join_coarse();
   Top3dSolveStats stats{};
   // live timing of the fine sweeps (top3d_sweep_timing)
   bool sweep_timing = false;
@@ -50,7 +59,8 @@ struct SolverState {
 
   bool allocate(DevGrid *grid, int nlev);
   void release();
-  bool setup(const double *x);
+  bool setup_main(const double *x);
+  bool setup_aux();
   void precondition();
   int solve(const double *x, const double *b, double *u, double tol, int *final_iter,
             float *final_res);

@@ -24,6 +24,15 @@ bool hip_ok(hipError_t e, const char *what);
 #define T3D_HIP(call) ::t3d::hip_ok((call), #call)
 
 hipStream_t stream();
+// While a StreamScope lives, stream() returns its stream on this thread.
+struct StreamScope {
+  explicit StreamScope(hipStream_t s);
+  ~StreamScope();
+  StreamScope(const StreamScope &) = delete;
+  StreamScope &operator=(const StreamScope &) = delete;
+  hipStream_t prev;
+  bool prev_on;
+};
 
 // ---------------------------------------------------------------------------
 // Geometry of one multigrid level in the halo layout: node (i,j,k) with
@@ -144,8 +153,9 @@ void launch_invdiag_stencil(const DevGrid &g, int l, const double *S, T *invD);
 void launch_elem_level2(const DevGrid &g, const double *E, double *Ke2);
 void launch_elem_galerkin(const DevGrid &g, int l, const double *Kchild,
                           double *Kparent);
+// S32: FP32 copy of S written alongside (optional)
 void launch_node_stencil(const DevGrid &g, int l, const double *Ke, double *S,
-                         int bc_symmetric);
+                         int bc_symmetric, float *S32 = nullptr);
 struct CoarseFactor;
 // assemble level l's stencil into cf.A (storage order, after begin_assembly)
 void launch_stencil_to_dense(const DevGrid &g, int l, const double *S, CoarseFactor &cf);
@@ -182,7 +192,11 @@ struct CoarseFactor {
   bool allocate(int n, int bw, bool f32, int planes);
   void release();
   void begin_assembly();  // A = 0 with identity padding
-  int factor();  // TOP3D_OK or TOP3D_ERR_NOT_SPD / TOP3D_ERR_HIP
+  // TOP3D_OK or TOP3D_ERR_NOT_SPD / TOP3D_ERR_HIP; f32out: W32 and WT32 for
+  // the FP32 apply instead of the FP64 WT
+  int factor(bool f32out = false);
+  void factor_async(bool f32out);  // the same work, queued without waiting
+  int check_info();                // after factor_async has completed
   template <typename T> const T *w() const {
     if constexpr (sizeof(T) == 8) return (const T *)W; else return (const T *)W32;
   }
