@@ -189,15 +189,21 @@ void launch_elem_galerkin(const DevGrid &g, int l, const double *Kchild, double 
 // bc_symmetric = 0 keeps the columns of fixed DOFs and only replaces fixed
 // rows by identity rows (the matrix-free convention, stencil_methods.c:515-518).
 // S32: an FP32 copy written alongside when given.
+// HALF: the symmetric half layout of a matrix with symmetric elimination
+// (kHalfPlanes planes): the 13 neighbours nb = 14..26 as full 3x3 blocks at
+// planes (nb-14)*9 + 3r + c, the centre's upper triangle r <= c at planes
+// 117 + half_centre_index(r, c); one thread per stored neighbour.
 constexpr int kNsNodes = 32;
 
+template <bool HALF>
 __global__ __launch_bounds__(27 * kNsNodes) void k_node_stencil(
     int nelx, int nely, int nelz, int wy, int wz, const double *__restrict__ Ke,
     const uint8_t *__restrict__ fix, double *__restrict__ S, float *__restrict__ S32,
     int bc_symmetric) {
-  __shared__ double tile[243 * kNsNodes];
+  constexpr int kPlanes = HALF ? kHalfPlanes : 243;
+  __shared__ double tile[kPlanes * kNsNodes];
   const int64_t nx = nelx + 1, ny = nely + 1, nz = nelz + 1, nint = nx * ny * nz;
-  const int nb = threadIdx.x;
+  const int nb = HALF ? 13 + threadIdx.x : threadIdx.x;
   const int64_t n0 = (int64_t)blockIdx.x * kNsNodes;
   const int64_t n = n0 + threadIdx.y;
   const int nloc = (int)min((int64_t)kNsNodes, nint - n0);
@@ -229,12 +235,23 @@ __global__ __launch_bounds__(27 * kNsNodes) void k_node_stencil(
           if (fr || fc) v[3 * r + c] = (nb == 13 && r == c) ? 1.0 : 0.0;
         }
     }
+    if (!HALF) {
 #pragma unroll
-    for (int q = 0; q < 9; q++) tile[(nb * 9 + q) * kNsNodes + threadIdx.y] = v[q];
+      for (int q = 0; q < 9; q++) tile[(nb * 9 + q) * kNsNodes + threadIdx.y] = v[q];
+    } else if (nb > 13) {
+#pragma unroll
+      for (int q = 0; q < 9; q++) tile[((nb - 14) * 9 + q) * kNsNodes + threadIdx.y] = v[q];
+    } else {
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = r; c < 3; c++)
+          tile[(117 + half_centre_index(r, c)) * kNsNodes + threadIdx.y] = v[3 * r + c];
+    }
   }
   __syncthreads();
-  const int tid = threadIdx.y * 27 + threadIdx.x;
-  for (int e = tid; e < 243 * kNsNodes; e += 27 * kNsNodes) {
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  for (int e = tid; e < kPlanes * kNsNodes; e += blockDim.x * kNsNodes) {
     const int q = e / kNsNodes, nn = e % kNsNodes;
     if (nn < nloc) {
       const double val = tile[e];
@@ -245,13 +262,16 @@ __global__ __launch_bounds__(27 * kNsNodes) void k_node_stencil(
 }
 
 void launch_node_stencil(const DevGrid &g, int l, const double *Ke, double *S,
-                         int bc_symmetric, float *S32) {
+                         int bc_symmetric, float *S32, bool half) {
   const Level &L = g.lv[l];
   const int64_t nint = L.interior_nodes();
-  hipLaunchKernelGGL(k_node_stencil, dim3((unsigned)((nint + kNsNodes - 1) / kNsNodes)),
-                     dim3(27, kNsNodes), 0, stream(),
-                     L.nelx, L.nely, L.nelz, L.wy, L.wz, Ke, g.fixmask[l], S, S32,
-                     bc_symmetric);
+  const dim3 grid((unsigned)((nint + kNsNodes - 1) / kNsNodes));
+  if (half)
+    hipLaunchKernelGGL((k_node_stencil<true>), grid, dim3(14, kNsNodes), 0, stream(), L.nelx,
+                       L.nely, L.nelz, L.wy, L.wz, Ke, g.fixmask[l], S, S32, 1);
+  else
+    hipLaunchKernelGGL((k_node_stencil<false>), grid, dim3(27, kNsNodes), 0, stream(), L.nelx,
+                       L.nely, L.nelz, L.wy, L.wz, Ke, g.fixmask[l], S, S32, bc_symmetric);
 }
 
 // the assembled coarsest level into the coarse factor's np x np storage

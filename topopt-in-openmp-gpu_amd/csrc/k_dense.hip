@@ -77,10 +77,11 @@ __device__ inline void store_acc(double *X, const double4_t acc[2][2], int qr, i
 // Cholesky and inverse of one 64x64 SPD tile in LDS, 256 threads (4 waves),
 // blocked by 16: per diagonal block kb
 //   (a) wave 0, lanes 0..15 (one row each, cross-lane reads): the 16x16
-//       Cholesky L_D and its inverse X_D;
-//   (b) the panel below: P <- P X_D^T;
+//       Cholesky L_D (reciprocal square roots refined by Newton steps) and
+//       its inverse X_D by forward substitution;
+//   (b) the panel below: P <- P X_D^T on MFMA;
 //   (c) the trailing lower triangle: C -= P P^T as 16x16 MFMA tiles;
-// then the off-diagonal 16x16 blocks of the inverse by distance.
+// then the off-diagonal 16x16 blocks of the inverse by distance on MFMA.
 // In: T (lower triangle; rows/cols beyond the matrix padded with identity).
 // Out: T = L (zeros above the diagonal), X = L^{-1}.  A non-positive pivot
 // sets *info (and is replaced by 1 so that the factorization finishes).
@@ -92,8 +93,19 @@ __device__ inline double readlane_d(double v, int lane) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-__device__ __forceinline__ void factor_tile(double *T, double *X, double *Ys, double *wbuf,
-                                            int *info) {
+// 1/sqrt(x) for x > 0: the hardware estimate and two Newton steps
+// y <- y + y (1 - x y^2) / 2
+__device__ inline double rsqrt_refined(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+  for (int it = 0; it < 2; it++) {
+    const double h = fma(-x * y, y, 1.0);
+    y = fma(0.5 * y, h, y);
+  }
+  return y;
+}
+
+__device__ __forceinline__ void factor_tile(double *T, double *X, int *info) {
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
   const int li = lane & 15, lk = lane >> 4;
   for (int e = t; e < kNB * kNB; e += 256) X[(e >> 6) * kLd + (e & 63)] = 0.0;
@@ -102,7 +114,7 @@ __device__ __forceinline__ void factor_tile(double *T, double *X, double *Ys, do
     const int d0 = 16 * kb;
     if (t < 16) {  // (a): one row per lane, broadcasts by v_readlane
       const int r = t;
-      double v[16];
+      double v[16], inv[16];
 #pragma unroll
       for (int c = 0; c < 16; c++) v[c] = (c <= r) ? T[(d0 + r) * kLd + d0 + c] : 0.0;
 #pragma unroll
@@ -112,9 +124,9 @@ __device__ __forceinline__ void factor_tile(double *T, double *X, double *Ys, do
           if (r == 0) info[0] = 1;
           piv = 1.0;
         }
-        const double ljj = sqrt(piv);
-        const double inv = 1.0 / ljj;
-        const double lrj = (r == j) ? ljj : v[j] * inv;
+        const double ij = rsqrt_refined(piv);
+        inv[j] = ij;  // 1 / L_jj
+        const double lrj = (r == j) ? piv * ij : v[j] * ij;
         v[j] = (r >= j) ? lrj : 0.0;
 #pragma unroll
         for (int c = j + 1; c < 16; c++) {
@@ -126,39 +138,31 @@ __device__ __forceinline__ void factor_tile(double *T, double *X, double *Ys, do
       for (int c = 0; c < 16; c++) T[(d0 + r) * kLd + d0 + c] = v[c];
       // column r of X_D = L_D^{-1}: forward substitution with the rows of
       // L_D broadcast from their lanes
-      double dinv[16], x[16];
-#pragma unroll
-      for (int i = 0; i < 16; i++) dinv[i] = 1.0 / readlane_d(v[i], i);
+      double x[16];
 #pragma unroll
       for (int i = 0; i < 16; i++) {
         double acc = (i == r) ? 1.0 : 0.0;
 #pragma unroll
         for (int m = 0; m < i; m++) acc -= readlane_d(v[m], i) * x[m];
-        x[i] = acc * dinv[i];
+        x[i] = acc * inv[i];
         X[(d0 + i) * kLd + d0 + r] = x[i];
       }
     }
     __syncthreads();
     const int npan = kNB - d0 - 16;  // panel rows below the diagonal block
     if (npan > 0) {
-      // (b) P[p][c] <- sum_{m <= c} P[p][m] X_D[c][m]
-      double pv[3];
+      // (b) P[p][c] <- sum_m P[p][m] X_D[c][m], one 16-row block per wave
+      const int p0 = d0 + 16 + 16 * w;
+      double4_t pacc = {0.0, 0.0, 0.0, 0.0};
+      if (16 * w < npan) {
 #pragma unroll
-      for (int q = 0; q < 3; q++) {
-        const int e = t + 256 * q;
-        pv[q] = 0.0;
-        if (e < npan * 16) {
-          const int p = d0 + 16 + (e >> 4), c = e & 15;
-          double acc = 0.0;
-          for (int m = 0; m <= c; m++) acc += T[p * kLd + d0 + m] * X[(d0 + c) * kLd + d0 + m];
-          pv[q] = acc;
-        }
+        for (int k = 0; k < 16; k += 4)
+          pacc = mfma4(T[(p0 + li) * kLd + d0 + k + lk], X[(d0 + li) * kLd + d0 + k + lk], pacc);
       }
       __syncthreads();
+      if (16 * w < npan) {
 #pragma unroll
-      for (int q = 0; q < 3; q++) {
-        const int e = t + 256 * q;
-        if (e < npan * 16) T[(d0 + 16 + (e >> 4)) * kLd + d0 + (e & 15)] = pv[q];
+        for (int v = 0; v < 4; v++) T[(p0 + lk + 4 * v) * kLd + d0 + li] = pacc[v];
       }
       __syncthreads();
       // (c) trailing tiles (ti, tj), kb < tj <= ti < 4, round-robin over waves
@@ -189,27 +193,25 @@ __device__ __forceinline__ void factor_tile(double *T, double *X, double *Ys, do
     if (c > r) T[r * kLd + c] = 0.0;
   }
   __syncthreads();
-  // off-diagonal blocks of the inverse by distance d:
-  //   X_IJ = -X_II sum_{K=J}^{I-1} L_IK X_KJ
-  const int rr = t >> 4, cc = t & 15;
+  // off-diagonal blocks of the inverse by distance d, one block per wave:
+  //   Y = sum_{K=J}^{I-1} L_IK X_KJ,  X_IJ = -X_II Y
+  // (the accumulator of Y is the B operand of the second product as is:
+  // register v of lane l holds Y[l/16 + 4v][l%16], the B element of k-step v)
   for (int d = 1; d < 4; d++) {
-    const int ntile = 4 - d;
-    for (int a = 0; a < ntile; a++) {
-      const int I = d + a, J = a;
-      double y = 0.0;
+    if (w < 4 - d) {
+      const int J = w, I = d + w;
+      double4_t y = {0.0, 0.0, 0.0, 0.0};
       for (int K = J; K < I; K++)
 #pragma unroll
-        for (int m = 0; m < 16; m++)
-          y += T[(16 * I + rr) * kLd + 16 * K + m] * X[(16 * K + m) * kLd + 16 * J + cc];
-      Ys[a * 256 + t] = y;
-    }
-    __syncthreads();
-    for (int a = 0; a < ntile; a++) {
-      const int I = d + a, J = a;
-      double xv = 0.0;
+        for (int k = 0; k < 16; k += 4)
+          y = mfma4(T[(16 * I + li) * kLd + 16 * K + k + lk], X[(16 * K + k + lk) * kLd + 16 * J + li],
+                    y);
+      double4_t o = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int m = 0; m < 16; m++) xv -= X[(16 * I + rr) * kLd + 16 * I + m] * Ys[a * 256 + m * 16 + cc];
-      X[(16 * I + rr) * kLd + 16 * J + cc] = xv;
+      for (int v = 0; v < 4; v++)
+        o = mfma4(X[(16 * I + li) * kLd + 16 * I + 4 * v + lk], y[v], o);
+#pragma unroll
+      for (int v = 0; v < 4; v++) X[(16 * I + lk + 4 * v) * kLd + 16 * J + li] = -o[v];
     }
     __syncthreads();
   }
@@ -243,8 +245,6 @@ __global__ __launch_bounds__(256) void k_chol_task(double *__restrict__ A, doubl
   __shared__ double PJ[kNB * kLd];
   __shared__ double DV[kNB * kLd];
   __shared__ double CT[kNB * kLd];
-  __shared__ double Ys[3 * 256];
-  __shared__ double wbuf[32];
   const CholTask tk = tasks[blockIdx.x];
   const int tid = threadIdx.x;
   const bool diag = tk.I == tk.J;
@@ -323,7 +323,7 @@ __global__ __launch_bounds__(256) void k_chol_task(double *__restrict__ A, doubl
     if (!fac) return;
   }
   __syncthreads();
-  factor_tile(DV, PJ, Ys, wbuf, info);
+  factor_tile(DV, PJ, info);
   store_diag(DV, PJ, Lf, Dinv, np, tk.I);
 }
 
@@ -389,22 +389,35 @@ __global__ __launch_bounds__(256) void k_linv_seq(const double *__restrict__ Lf,
 #pragma unroll
     for (int t = 0; t < 16; t++)
       dreg[t] = (4 * t < 16 * (w + 1)) ? D[(16 * w + li) * kNB + 4 * t + lk] : 0.0;
-    double treg[16];
+    // this lane's B elements of W_K: rows k + lk (k = 0, 4, .., 60), column gc
+    auto load_b = [&](int K, double (&breg)[16]) {
+      const double *wk = W + (int64_t)K * kNB * np + gc;
+#pragma unroll
+      for (int q = 0; q < 16; q++) breg[q] = wk[(int64_t)(4 * q + lk) * np];
+    };
+    double treg[16], bcur[16], bnxt[16];
     load_tile(I, klist[rw.y], treg);
+    load_b(klist[rw.y], bcur);
     store_tile(Lt, treg);
     __syncthreads();
     double4_t acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
     for (int t = 0; t < rw.z; t++) {
       const double *cur = Lt + (size_t)(t & 1) * kNB * kLd;
       double *nxt = Lt + (size_t)((t & 1) ^ 1) * kNB * kLd;
-      if (t + 1 < rw.z) load_tile(I, klist[rw.y + t + 1], treg);
-      const double *wk = W + (int64_t)klist[rw.y + t] * kNB * np + gc;
-#pragma unroll
-      for (int k = 0; k < kNB; k += 8) {
-        acc0 = mfma4(cur[(16 * w + li) * kLd + k + lk], wk[(int64_t)(k + lk) * np], acc0);
-        acc1 = mfma4(cur[(16 * w + li) * kLd + k + 4 + lk], wk[(int64_t)(k + 4 + lk) * np], acc1);
+      if (t + 1 < rw.z) {  // the next L_IK and W_K in flight during this product
+        load_tile(I, klist[rw.y + t + 1], treg);
+        load_b(klist[rw.y + t + 1], bnxt);
       }
-      if (t + 1 < rw.z) store_tile(nxt, treg);
+#pragma unroll
+      for (int q = 0; q < 16; q += 2) {
+        acc0 = mfma4(cur[(16 * w + li) * kLd + 4 * q + lk], bcur[q], acc0);
+        acc1 = mfma4(cur[(16 * w + li) * kLd + 4 * q + 4 + lk], bcur[q + 1], acc1);
+      }
+      if (t + 1 < rw.z) {
+        store_tile(nxt, treg);
+#pragma unroll
+        for (int q = 0; q < 16; q++) bcur[q] = bnxt[q];
+      }
       __syncthreads();
     }
 #pragma unroll

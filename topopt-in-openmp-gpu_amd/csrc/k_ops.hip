@@ -392,6 +392,85 @@ void launch_stencil_rows(const DevGrid &g, int l, int mode, const T *S, const T 
                        L.nz, L.wy, L.wz, S, in, out, b, invD, omega);
 }
 
+// Solver variant on the symmetric half layout (launch_node_stencil, half):
+// one thread per (interior node n, row r).  Row r of the block towards an
+// upper neighbour nb > 13 is stored at n; towards a lower neighbour
+// nb < 13 it is column r of the block that the neighbour n' stores towards
+// n (its upper neighbour 26 - nb), read there when n' is an interior node
+// (otherwise the block is zero).
+// 32-bit offsets: the host checks kHalfPlanes * nint and 3 * nodes < 2^31.
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_stencil_half(
+    int nx, int ny, int nz, int wy, int wz, const T *__restrict__ Sh,
+    const T *__restrict__ in, T *__restrict__ out, const T *__restrict__ b,
+    const T *__restrict__ invD, T omega) {
+  const uint32_t nint = (uint32_t)nx * ny * nz;
+  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  if (n >= nint) return;
+  const uint32_t nyz = (uint32_t)ny * nz;
+  const int i = (int)(n / nyz), k = (int)((n - i * nyz) / ny), j = (int)(n - i * nyz - k * ny);
+  const uint32_t node = ((uint32_t)(i + 1) * wz + (k + 1)) * wy + (j + 1);
+  T acc = 0;
+#pragma unroll
+  for (int dx = -1; dx <= 1; dx++)
+#pragma unroll
+    for (int dz = -1; dz <= 1; dz++)
+#pragma unroll
+      for (int dy = -1; dy <= 1; dy++) {
+        const int nbi = (dx + 1) * 9 + (dz + 1) * 3 + (dy + 1);
+        const uint32_t q = node + (dx * wz + dz) * wy + dy;
+        T s0, s1, s2;
+        if (nbi > 13) {
+          const T *s = Sh + ((nbi - 14) * 9 + 3 * r) * nint + n;
+          s0 = s[0];
+          s1 = s[nint];
+          s2 = s[2 * nint];
+        } else if (nbi == 13) {  // symmetric centre: C[r][c] = C[min][max]
+          s0 = Sh[(117 + half_centre_index(0, r)) * nint + n];
+          s1 = Sh[(117 + half_centre_index(r < 1 ? r : 1, r < 1 ? 1 : r)) * nint + n];
+          s2 = Sh[(117 + half_centre_index(r, 2)) * nint + n];
+        } else {
+          const bool inside = i + dx >= 0 && i + dx < nx && j + dy >= 0 && j + dy < ny &&
+                              k + dz >= 0 && k + dz < nz;
+          s0 = s1 = s2 = T(0);
+          if (inside) {  // column r of the block neighbour n2 stores towards n
+            const uint32_t n2 = n + (dx * nz + dz) * ny + dy;
+            const T *s = Sh + ((12 - nbi) * 9 + r) * nint + n2;
+            s0 = s[0];
+            s1 = s[3 * nint];
+            s2 = s[6 * nint];
+          }
+        }
+        acc += s0 * in[3 * q] + s1 * in[3 * q + 1] + s2 * in[3 * q + 2];
+      }
+  const uint32_t o = 3 * node + r;
+  const T bb = MODE == kApply ? T(0) : b[o];
+  const T dd = MODE == kJacobi ? invD[o] : T(0);
+  out[o] = finish<T>(MODE, acc, in[o], bb, dd, omega, false);
+}
+
+template <typename T>
+void launch_stencil_half(const DevGrid &g, int l, int mode, const T *Sh, const T *in, T *out,
+                         const T *b, const T *invD, T omega) {
+  const Level &L = g.lv[l];
+  const int64_t nint = L.interior_nodes();
+  if (kHalfPlanes * nint >= (int64_t(1) << 31) || 3 * L.nodes() >= (int64_t(1) << 31)) {
+    set_error(TOP3D_ERR_ARG, "level %d too large for 32-bit stencil offsets", l);
+    return;
+  }
+  dim3 bs(256), gs((unsigned)((nint + 255) / 256), 3);
+  if (mode == kApply)
+    hipLaunchKernelGGL((k_stencil_half<T, kApply>), gs, bs, 0, stream(), L.nx, L.ny, L.nz,
+                       L.wy, L.wz, Sh, in, out, b, invD, omega);
+  else if (mode == kJacobi)
+    hipLaunchKernelGGL((k_stencil_half<T, kJacobi>), gs, bs, 0, stream(), L.nx, L.ny, L.nz,
+                       L.wy, L.wz, Sh, in, out, b, invD, omega);
+  else
+    hipLaunchKernelGGL((k_stencil_half<T, kResidual>), gs, bs, 0, stream(), L.nx, L.ny,
+                       L.nz, L.wy, L.wz, Sh, in, out, b, invD, omega);
+}
+
 template <typename T>
 void launch_stencil(const DevGrid &g, int l, int mode, const T *S, const T *in,
                     T *out, const T *b, const T *invD, T omega) {
@@ -624,6 +703,33 @@ __global__ void k_invdiag_stencil(int nx, int ny, int nz, int wy, int wz,
     invD[3 * node + c] = (T)(1.0 / S[(int64_t)(centre * 9 + 4 * c) * nint + n]);
 }
 
+// inverse diagonal from a half-layout stencil (centre entries (c, c))
+template <typename T>
+__global__ void k_invdiag_half(int nx, int ny, int nz, int wy, int wz,
+                               const double *__restrict__ Sh, T *__restrict__ invD) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y * blockDim.y + threadIdx.y;
+  const int i = blockIdx.z;
+  if (j >= wy || k >= wz) return;
+  const int64_t node = ((int64_t)i * wz + k) * wy + j;
+  if (!(i >= 1 && i <= nx && j >= 1 && j <= ny && k >= 1 && k <= nz)) {
+    invD[3 * node] = invD[3 * node + 1] = invD[3 * node + 2] = T(0);
+    return;
+  }
+  const int64_t nint = (int64_t)nx * ny * nz;
+  const int64_t n = ((int64_t)(i - 1) * nz + (k - 1)) * ny + (j - 1);
+  for (int c = 0; c < 3; c++)
+    invD[3 * node + c] = (T)(1.0 / Sh[(int64_t)(117 + half_centre_index(c, c)) * nint + n]);
+}
+
+template <typename T>
+void launch_invdiag_half(const DevGrid &g, int l, const double *Sh, T *invD) {
+  const Level &L = g.lv[l];
+  dim3 bs(64, 4, 1), gs((L.wy + 63) / 64, (L.wz + 3) / 4, L.wx);
+  hipLaunchKernelGGL((k_invdiag_half<T>), gs, bs, 0, stream(), L.nx, L.ny, L.nz, L.wy, L.wz,
+                     Sh, invD);
+}
+
 template <typename T>
 void launch_invdiag_stencil(const DevGrid &g, int l, const double *S, T *invD) {
   const Level &L = g.lv[l];
@@ -645,6 +751,9 @@ void launch_invdiag_stencil(const DevGrid &g, int l, const double *S, T *invD) {
                                   const T *, const T *, T);                              \
   template void launch_stencil_rows<T>(const DevGrid &, int, int, const T *, const T *,  \
                                        T *, const T *, const T *, T);                    \
+  template void launch_stencil_half<T>(const DevGrid &, int, int, const T *, const T *,  \
+                                       T *, const T *, const T *, T);                    \
+  template void launch_invdiag_half<T>(const DevGrid &, int, const double *, T *);       \
   template void launch_restrict<T>(const DevGrid &, int, const T *, T *);                \
   template void launch_prolong_add<T>(const DevGrid &, int, const T *, T *);             \
   template void launch_prolong<T>(const DevGrid &, int, const T *, T *);                 \

@@ -82,7 +82,8 @@ bool SolverState::allocate(DevGrid *grid, int nlev) {
   if (!T3D_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking)) ||
       !T3D_HIP(hipEventCreateWithFlags(&ev_base, hipEventDisableTiming)) ||
       !T3D_HIP(hipEventCreateWithFlags(&ev_stencil, hipEventDisableTiming)) ||
-      !T3D_HIP(hipEventCreateWithFlags(&ev_coarse, hipEventDisableTiming)))
+      !T3D_HIP(hipEventCreateWithFlags(&ev_coarse, hipEventDisableTiming)) ||
+      !T3D_HIP(hipEventCreateWithFlags(&ev_done, hipEventDisableTiming)))
     return false;
   return T3D_HIP(hipStreamSynchronize(stream())) && r && p && q && z;
 }
@@ -109,7 +110,7 @@ void SolverState::release() {
   f(scal); f(partials); f(cgpart);
   if (hscal) hipHostFree(hscal);
   if (ev) hipEventDestroy(ev);
-  for (hipEvent_t e : {ev_base, ev_stencil, ev_coarse})
+  for (hipEvent_t e : {ev_base, ev_stencil, ev_coarse, ev_done})
     if (e) hipEventDestroy(e);
   if (aux) hipStreamDestroy(aux);
   aux = nullptr;
@@ -145,10 +146,15 @@ bool SolverState::setup_aux() {
         launch_elem_level2(*g, E64, Ke[2]);
       else
         launch_elem_galerkin(*g, l, Ke[l - 1], Ke[l]);
-      // the smoother levels of the mixed V-cycle also get the FP32 stencil
+      // smoother levels: the symmetric half layout (and its FP32 copy for
+      // the mixed V-cycle); the coarsest level: the full stencil
       const bool smoother = l < nl - 1;
-      launch_node_stencil(*g, l, Ke[l], S64[l], 1, (mixed && smoother) ? S32[l] : nullptr);
-      launch_invdiag_stencil<double>(*g, l, S64[l], invD64[l]);
+      launch_node_stencil(*g, l, Ke[l], S64[l], 1, (mixed && smoother) ? S32[l] : nullptr,
+                          smoother);
+      if (smoother)
+        launch_invdiag_half<double>(*g, l, S64[l], invD64[l]);
+      else
+        launch_invdiag_stencil<double>(*g, l, S64[l], invD64[l]);
       if (mixed && smoother) launch_convert<float, double>(g->lv[l].ndof, invD64[l], invD32[l]);
       if (l == nl - 2) hipEventRecord(ev_stencil, aux);
     }
@@ -156,6 +162,10 @@ bool SolverState::setup_aux() {
     launch_stencil_to_dense(*g, nl - 1, S64[nl - 1], coarse);
     coarse.factor_async(mixed);
     if (!T3D_HIP(hipEventRecord(ev_coarse, aux))) return false;
+    // the fine inverse diagonal is not read by the fine sweeps when they form
+    // it from E; it is still filled in for callers of the SolverDataBuffer
+    if (fine_sweep_forms_diagonal()) launch_invdiag_mf<double>(*g, 0, E64, invD64[0]);
+    if (!T3D_HIP(hipEventRecord(ev_done, aux))) return false;
   }
   wait_stencil = wait_coarse = true;
   return true;
@@ -199,7 +209,7 @@ void SolverState::op(int l, int mode, const T *in, T *out, const T *b) {
   else if (l == 1)
     launch_level1<T>(*g, mode, young_t<T>(), in, out, b, invD, omega, bcsym);
   else
-    launch_stencil_rows<T>(*g, l, mode, stencil_t<T>(l), in, out, b, invD, omega);
+    launch_stencil_half<T>(*g, l, mode, stencil_t<T>(l), in, out, b, invD, omega);
 }
 
 // nswp damped-Jacobi sweeps on level l starting from z = 0 (first sweep
@@ -293,8 +303,9 @@ int SolverState::solve(const double *x, const double *b, double *u, double tol, 
   hipMemcpyAsync(hscal + 4, scal + 4, sizeof(double), hipMemcpyDeviceToHost, stream());
   // the coarse set-up is queued while the library stream computes r
   if (!setup_aux()) {
-    join_coarse();
+    hipStreamSynchronize(aux);
     hipStreamSynchronize(stream());
+    wait_stencil = wait_coarse = false;
     stats.status = top3d_last_error();
     return stats.status;
   }
@@ -349,7 +360,8 @@ int SolverState::solve(const double *x, const double *b, double *u, double tol, 
     }
     if (relres < tol) break;
   }
-  join_coarse();  // the factorization has finished before the solve returns
+  join_coarse();  // the work on `aux` has finished before the solve returns
+  hipStreamWaitEvent(stream(), ev_done, 0);
   if (!T3D_HIP(hipStreamSynchronize(stream()))) status = TOP3D_ERR_HIP;
   if (status == TOP3D_OK) {
     const int rc = coarse.check_info();

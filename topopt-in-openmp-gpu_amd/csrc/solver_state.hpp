@@ -44,7 +44,7 @@ struct SolverState {
   // waits for ev_stencil before its first sweep on level 2 and for ev_coarse
   // before its first coarse solve.
   hipStream_t aux = nullptr;
-  hipEvent_t ev_base = nullptr, ev_stencil = nullptr, ev_coarse = nullptr;
+  hipEvent_t ev_base = nullptr, ev_stencil = nullptr, ev_coarse = nullptr, ev_done = nullptr;
   bool wait_stencil = false, wait_coarse = false;
   void join_stencil();
   void join_coarse();

@@ -153,9 +153,24 @@ void launch_invdiag_stencil(const DevGrid &g, int l, const double *S, T *invD);
 void launch_elem_level2(const DevGrid &g, const double *E, double *Ke2);
 void launch_elem_galerkin(const DevGrid &g, int l, const double *Kchild,
                           double *Kparent);
-// S32: FP32 copy of S written alongside (optional)
+// S32: FP32 copy of S written alongside (optional).  half: the symmetric
+// half layout (kHalfPlanes planes, symmetric elimination of fixed DOFs):
+// neighbours nb = 14..26 at planes (nb-14)*9 + 3r + c, the centre's entries
+// r <= c at planes 117 + half_centre_index(r, c)
+constexpr int kHalfPlanes = 123;
+__host__ __device__ constexpr int half_centre_index(int r, int c) {
+  return r == 0 ? c : (r == 1 ? 2 + c : 5);  // (0,0..2) -> 0..2, (1,1..2) -> 3..4, (2,2) -> 5
+}
 void launch_node_stencil(const DevGrid &g, int l, const double *Ke, double *S,
-                         int bc_symmetric, float *S32 = nullptr);
+                         int bc_symmetric, float *S32 = nullptr, bool half = false);
+// smoother sweep with a half-layout stencil (interior nodes only, as
+// launch_stencil_rows)
+template <typename T>
+void launch_stencil_half(const DevGrid &g, int l, int mode, const T *Sh, const T *in, T *out,
+                         const T *b, const T *invD, T omega);
+// inverse diagonal from a half-layout stencil
+template <typename T>
+void launch_invdiag_half(const DevGrid &g, int l, const double *Sh, T *invD);
 struct CoarseFactor;
 // assemble level l's stencil into cf.A (storage order, after begin_assembly)
 void launch_stencil_to_dense(const DevGrid &g, int l, const double *S, CoarseFactor &cf);
