@@ -352,6 +352,12 @@ __device__ __forceinline__ void tile_sweep_body(
   // 32-bit offsets: the host checks 3 * nodes < 2^31
   const uint32_t sx = (uint32_t)a.wy * (uint32_t)a.wz;
   const uint32_t n00 = (uint32_t)k * a.wy + j;  // node (j, k) within a plane
+  // Threads whose element column lies outside the grid read the halo node
+  // (0, 0) of each plane and the halo element (i, 0, 0) instead, both in
+  // bounds; that element's modulus is 0, so their products are 0 and no
+  // branch guards the loads.
+  const uint32_t n00c = elem_ok ? n00 : 0u;
+  const uint32_t e00c = elem_ok ? (uint32_t)k * a.few + j : 0u;
   const uint32_t noff[2][2] = {{0u, (uint32_t)a.wy}, {1u, (uint32_t)a.wy + 1u}};  // [py][pz]
   double dacc = 0.0;  // apply mode with dotp: partial sum of in . out
   // kDiag: moduli of the 4 elements of the previous slab at the owned node
@@ -365,19 +371,15 @@ __device__ __forceinline__ void tile_sweep_body(
     for (int py = 0; py < 2; py++)
 #pragma unroll
       for (int pz = 0; pz < 2; pz++) {
-        T v0 = 0, v1 = 0, v2 = 0;
-        if (elem_ok && i <= a.nelx + 2) {
-          const uint32_t nd = (uint32_t)i * sx + n00 + noff[py][pz];
-          const T *pn = in + 3u * nd;
-          v0 = pn[0];
-          v1 = pn[1];
-          v2 = pn[2];
-          if (BCSYM) {
-            const uint8_t m = fix[nd];
-            if (m & 1) v0 = 0;
-            if (m & 2) v1 = 0;
-            if (m & 4) v2 = 0;
-          }
+        // (the march only visits planes i <= nelx + 2)
+        const uint32_t nd = (uint32_t)i * sx + n00c + (elem_ok ? noff[py][pz] : 0u);
+        const T *pn = in + 3u * nd;
+        T v0 = pn[0], v1 = pn[1], v2 = pn[2];
+        if (BCSYM) {
+          const uint8_t m = fix[nd];
+          if (m & 1) v0 = 0;
+          if (m & 2) v1 = 0;
+          if (m & 4) v2 = 0;
         }
         v[py][pz][0] = v0;
         v[py][pz][1] = v1;
@@ -410,7 +412,8 @@ __device__ __forceinline__ void tile_sweep_body(
     T g[8][3];
     T Ee = 0;
     if constexpr (LVL == 0) {
-      if (elem_ok && i <= a.nelx + 1) Ee = E[((uint32_t)i * a.fez + k) * a.few + j];
+      // (the march only visits slabs i <= nelx + 1)
+      Ee = E[(uint32_t)i * a.fez * a.few + e00c];
       element_core<T>(B, Ee, h, g);
       if constexpr (kDiag) {
         if (i < i0 && own_ok) {  // first slab: the neighbours' moduli from memory
