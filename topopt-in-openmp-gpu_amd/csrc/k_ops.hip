@@ -398,7 +398,14 @@ void launch_stencil_rows(const DevGrid &g, int l, int mode, const T *S, const T 
 // nb < 13 it is column r of the block that the neighbour n' stores towards
 // n (its upper neighbour 26 - nb), read there when n' is an interior node
 // (otherwise the block is zero).
-// 32-bit offsets: the host checks kHalfPlanes * nint and 3 * nodes < 2^31.
+// Addresses are a wave-uniform base (plane, neighbour offset) plus the
+// thread's 32-bit byte offset, which maps onto scalar-base + vector-offset
+// loads.  The host checks the byte sizes of Sh and `in` < 2^31.
+template <typename T>
+__device__ __forceinline__ T ld_at(const T *base, uint32_t byte_off) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_stencil_half(
     int nx, int ny, int nz, int wy, int wz, const T *__restrict__ Sh,
@@ -411,6 +418,8 @@ __global__ __launch_bounds__(256) void k_stencil_half(
   const uint32_t nyz = (uint32_t)ny * nz;
   const int i = (int)(n / nyz), k = (int)((n - i * nyz) / ny), j = (int)(n - i * nyz - k * ny);
   const uint32_t node = ((uint32_t)(i + 1) * wz + (k + 1)) * wy + (j + 1);
+  const uint32_t sb = (uint32_t)sizeof(T) * n;         // byte offset in a stencil plane
+  const uint32_t ub = (uint32_t)sizeof(T) * 3 * node;  // byte offset of the node's values
   T acc = 0;
 #pragma unroll
   for (int dx = -1; dx <= 1; dx++)
@@ -419,30 +428,30 @@ __global__ __launch_bounds__(256) void k_stencil_half(
 #pragma unroll
       for (int dy = -1; dy <= 1; dy++) {
         const int nbi = (dx + 1) * 9 + (dz + 1) * 3 + (dy + 1);
-        const uint32_t q = node + (dx * wz + dz) * wy + dy;
+        const T *uq = in + 3 * (int64_t)((dx * wz + dz) * wy + dy);  // neighbour q of the node
         T s0, s1, s2;
         if (nbi > 13) {
-          const T *s = Sh + ((nbi - 14) * 9 + 3 * r) * nint + n;
-          s0 = s[0];
-          s1 = s[nint];
-          s2 = s[2 * nint];
+          const T *s = Sh + (int64_t)((nbi - 14) * 9 + 3 * r) * nint;
+          s0 = ld_at(s, sb);
+          s1 = ld_at(s + nint, sb);
+          s2 = ld_at(s + 2 * (int64_t)nint, sb);
         } else if (nbi == 13) {  // symmetric centre: C[r][c] = C[min][max]
-          s0 = Sh[(117 + half_centre_index(0, r)) * nint + n];
-          s1 = Sh[(117 + half_centre_index(r < 1 ? r : 1, r < 1 ? 1 : r)) * nint + n];
-          s2 = Sh[(117 + half_centre_index(r, 2)) * nint + n];
+          const T *c = Sh + (int64_t)117 * nint;
+          s0 = ld_at(c + (int64_t)half_centre_index(0, r) * nint, sb);
+          s1 = ld_at(c + (int64_t)half_centre_index(r < 1 ? r : 1, r < 1 ? 1 : r) * nint, sb);
+          s2 = ld_at(c + (int64_t)half_centre_index(r, 2) * nint, sb);
         } else {
           const bool inside = i + dx >= 0 && i + dx < nx && j + dy >= 0 && j + dy < ny &&
                               k + dz >= 0 && k + dz < nz;
           s0 = s1 = s2 = T(0);
-          if (inside) {  // column r of the block neighbour n2 stores towards n
-            const uint32_t n2 = n + (dx * nz + dz) * ny + dy;
-            const T *s = Sh + ((12 - nbi) * 9 + r) * nint + n2;
-            s0 = s[0];
-            s1 = s[3 * nint];
-            s2 = s[6 * nint];
+          if (inside) {  // column r of the block neighbour n + dn stores towards n
+            const T *s = Sh + (int64_t)((12 - nbi) * 9 + r) * nint + ((dx * nz + dz) * ny + dy);
+            s0 = ld_at(s, sb);
+            s1 = ld_at(s + 3 * (int64_t)nint, sb);
+            s2 = ld_at(s + 6 * (int64_t)nint, sb);
           }
         }
-        acc += s0 * in[3 * q] + s1 * in[3 * q + 1] + s2 * in[3 * q + 2];
+        acc += s0 * ld_at(uq, ub) + s1 * ld_at(uq + 1, ub) + s2 * ld_at(uq + 2, ub);
       }
   const uint32_t o = 3 * node + r;
   const T bb = MODE == kApply ? T(0) : b[o];
@@ -455,7 +464,8 @@ void launch_stencil_half(const DevGrid &g, int l, int mode, const T *Sh, const T
                          const T *b, const T *invD, T omega) {
   const Level &L = g.lv[l];
   const int64_t nint = L.interior_nodes();
-  if (kHalfPlanes * nint >= (int64_t(1) << 31) || 3 * L.nodes() >= (int64_t(1) << 31)) {
+  if ((int64_t)sizeof(T) * kHalfPlanes * nint >= (int64_t(1) << 31) ||
+      (int64_t)sizeof(T) * 3 * L.nodes() >= (int64_t(1) << 31)) {
     set_error(TOP3D_ERR_ARG, "level %d too large for 32-bit stencil offsets", l);
     return;
   }

@@ -194,7 +194,9 @@ void SolverState::op(int l, int mode, const T *in, T *out, const T *b) {
   if (l == 0 && mode == 1 && sweep_timing) {
     while (sw_ev.size() < sw_used + 2) {
       hipEvent_t e = nullptr;
-      if (!T3D_HIP(hipEventCreate(&e))) break;
+      // timing only: no system-scope fence (cache write-back) at the record,
+      // which would otherwise stall the stream between the timed sweeps
+      if (!T3D_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence))) break;
       sw_ev.push_back(e);
     }
     const bool rec = sw_ev.size() >= sw_used + 2;
