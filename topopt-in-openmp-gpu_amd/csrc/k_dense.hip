@@ -333,28 +333,28 @@ __global__ __launch_bounds__(256) void k_chol_task(double *__restrict__ A, doubl
 // J of W is non-zero, in increasing order:
 //   W_I = -L_II^{-1} sum_{K in ks(I)} L_IK W_K,
 // ks(I) the strip's earlier rows with L_IK != 0 (row and K lists from the
-// elimination schedule).  W_K is read back from W (written by this
-// workgroup before a barrier); the L_IK tiles are staged through LDS with the
-// next tile's loads in flight during the current product; the rows of
-// L_II^{-1} a wave needs are loaded at the start of the row.  W is always
-// written; WT, W32 and WT32 (FP32 copies) when given.
-__global__ __launch_bounds__(256) void k_linv_seq(const double *__restrict__ Lf,
-                                                  const double *__restrict__ Dinv, int np,
-                                                  const int2 *__restrict__ colinfo,
-                                                  const int4 *__restrict__ rows,
-                                                  const int *__restrict__ klist,
-                                                  double *__restrict__ W,
-                                                  double *__restrict__ WT,
-                                                  float *__restrict__ W32,
-                                                  float *__restrict__ WT32) {
-  __shared__ double Bx[kNB * 17];
-  __shared__ double Lt[2 * kNB * kLd];
-  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+// elimination schedule).  Two groups of 4 waves take alternate K of a row,
+// each with its own double-buffered LDS stage for the L_IK tiles (the next
+// tile in flight during the current product); their 64x16 partial sums meet
+// in LDS.  W_K is read back from W (written by this workgroup
+// before a barrier).  W is always written; WT, W32 and WT32 (FP32 copies)
+// when given.
+constexpr int kLinvGroups = 2;
+
+__global__ __launch_bounds__(256 * kLinvGroups) void k_linv_seq(
+    const double *__restrict__ Lf, const double *__restrict__ Dinv, int np,
+    const int2 *__restrict__ colinfo, const int4 *__restrict__ rows,
+    const int *__restrict__ klist, double *__restrict__ W, double *__restrict__ WT,
+    float *__restrict__ W32, float *__restrict__ WT32) {
+  __shared__ double Bx[kLinvGroups][kNB * 17];
+  __shared__ double Lt[kLinvGroups][2 * kNB * kLd];
+  const int tid = threadIdx.x, grp = tid >> 8, gtid = tid & 255;
+  const int w = gtid >> 6, lane = tid & 63;
   const int li = lane & 15, lk = lane >> 4;
   const int c0 = blockIdx.x * kStrip;
   const int J = c0 / kNB, cofs = c0 - J * kNB;
   const int gc = c0 + li;
-  for (int e = tid; e < kNB * kStrip; e += 256) {
+  for (int e = tid; e < kNB * kStrip; e += 256 * kLinvGroups) {
     const int r = e >> 4, j = e & 15;
     const double v = Dinv[(int64_t)J * kNB * kNB + r * kNB + cofs + j];
     const int64_t gr = (int64_t)J * kNB + r;
@@ -368,74 +368,80 @@ __global__ __launch_bounds__(256) void k_linv_seq(const double *__restrict__ Lf,
   auto load_tile = [&](int I, int K, double (&reg)[16]) {
 #pragma unroll
     for (int q = 0; q < 16; q++) {
-      const int e = tid + 256 * q;
+      const int e = gtid + 256 * q;
       reg[q] = Lf[((int64_t)I * kNB + (e >> 6)) * np + (int64_t)K * kNB + (e & 63)];
     }
   };
   auto store_tile = [&](double *dst, const double (&reg)[16]) {
 #pragma unroll
     for (int q = 0; q < 16; q++) {
-      const int e = tid + 256 * q;
+      const int e = gtid + 256 * q;
       dst[(e >> 6) * kLd + (e & 63)] = reg[q];
     }
   };
+  // this lane's B elements of W_K: rows k + lk (k = 0, 4, .., 60), column gc
+  auto load_b = [&](int K, double (&breg)[16]) {
+    const double *wk = W + (int64_t)K * kNB * np + gc;
+#pragma unroll
+    for (int q = 0; q < 16; q++) breg[q] = wk[(int64_t)(4 * q + lk) * np];
+  };
+  double *Lg = Lt[grp];
   __syncthreads();
   const int2 ci = colinfo[J];
   for (int q = 0; q < ci.y; q++) {
     const int4 rw = rows[ci.x + q];  // (I, offset into klist, count, -)
-    const int I = rw.x;
-    const double *D = Dinv + (int64_t)I * kNB * kNB;
-    double dreg[16];  // rows 16w.. of L_II^{-1}: columns k < 16(w+1) are non-zero
+    const int I = rw.x, nk = rw.z;
+    double dreg[16];  // group 0: rows 16w.. of L_II^{-1}, columns k < 16(w+1)
+    if (grp == 0) {
+      const double *D = Dinv + (int64_t)I * kNB * kNB;
 #pragma unroll
-    for (int t = 0; t < 16; t++)
-      dreg[t] = (4 * t < 16 * (w + 1)) ? D[(16 * w + li) * kNB + 4 * t + lk] : 0.0;
-    // this lane's B elements of W_K: rows k + lk (k = 0, 4, .., 60), column gc
-    auto load_b = [&](int K, double (&breg)[16]) {
-      const double *wk = W + (int64_t)K * kNB * np + gc;
-#pragma unroll
-      for (int q = 0; q < 16; q++) breg[q] = wk[(int64_t)(4 * q + lk) * np];
-    };
-    double treg[16], bcur[16], bnxt[16];
-    load_tile(I, klist[rw.y], treg);
-    load_b(klist[rw.y], bcur);
-    store_tile(Lt, treg);
+      for (int t = 0; t < 16; t++)
+        dreg[t] = (4 * t < 16 * (w + 1)) ? D[(16 * w + li) * kNB + 4 * t + lk] : 0.0;
+    }
+    double treg[16], bcur[16];
+    if (grp < nk) {  // this group's first K: t = grp
+      load_tile(I, klist[rw.y + grp], treg);
+      store_tile(Lg, treg);
+    }
     __syncthreads();
     double4_t acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
-    for (int t = 0; t < rw.z; t++) {
-      const double *cur = Lt + (size_t)(t & 1) * kNB * kLd;
-      double *nxt = Lt + (size_t)((t & 1) ^ 1) * kNB * kLd;
-      if (t + 1 < rw.z) {  // the next L_IK and W_K in flight during this product
-        load_tile(I, klist[rw.y + t + 1], treg);
-        load_b(klist[rw.y + t + 1], bnxt);
-      }
+    const int iters = (nk + kLinvGroups - 1) / kLinvGroups;
+    for (int it = 0; it < iters; it++) {
+      const int t = kLinvGroups * it + grp;  // this group's K-term
+      const bool has = t < nk, more = t + kLinvGroups < nk;
+      const double *cur = Lg + (size_t)(it & 1) * kNB * kLd;
+      double *nxt = Lg + (size_t)((it & 1) ^ 1) * kNB * kLd;
+      if (more) load_tile(I, klist[rw.y + t + kLinvGroups], treg);  // in flight meanwhile
+      if (has) {
+        load_b(klist[rw.y + t], bcur);
 #pragma unroll
-      for (int q = 0; q < 16; q += 2) {
-        acc0 = mfma4(cur[(16 * w + li) * kLd + 4 * q + lk], bcur[q], acc0);
-        acc1 = mfma4(cur[(16 * w + li) * kLd + 4 * q + 4 + lk], bcur[q + 1], acc1);
+        for (int qq = 0; qq < 16; qq += 2) {
+          acc0 = mfma4(cur[(16 * w + li) * kLd + 4 * qq + lk], bcur[qq], acc0);
+          acc1 = mfma4(cur[(16 * w + li) * kLd + 4 * qq + 4 + lk], bcur[qq + 1], acc1);
+        }
       }
-      if (t + 1 < rw.z) {
-        store_tile(nxt, treg);
-#pragma unroll
-        for (int q = 0; q < 16; q++) bcur[q] = bnxt[q];
-      }
+      if (more) store_tile(nxt, treg);
       __syncthreads();
     }
 #pragma unroll
-    for (int v = 0; v < 4; v++) Bx[(16 * w + lk + 4 * v) * 17 + li] = acc0[v] + acc1[v];
+    for (int v = 0; v < 4; v++) Bx[grp][(16 * w + lk + 4 * v) * 17 + li] = acc0[v] + acc1[v];
     __syncthreads();
-    double4_t o = {0.0, 0.0, 0.0, 0.0};
+    if (grp == 0) {
+      double4_t o = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int t = 0; t < 16; t++)
-      if (4 * t < 16 * (w + 1)) o = mfma4(dreg[t], Bx[(4 * t + lk) * 17 + li], o);
+      for (int t = 0; t < 16; t++)
+        if (4 * t < 16 * (w + 1))
+          o = mfma4(dreg[t], Bx[0][(4 * t + lk) * 17 + li] + Bx[1][(4 * t + lk) * 17 + li], o);
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-      const int64_t gr = (int64_t)I * kNB + 16 * w + lk + 4 * v;
-      const double val = -o[v];
-      W[gr * np + gc] = val;
-      if (WT) WT[(int64_t)gc * np + gr] = val;
-      if (W32) {
-        W32[gr * np + gc] = (float)val;
-        WT32[(int64_t)gc * np + gr] = (float)val;
+      for (int v = 0; v < 4; v++) {
+        const int64_t gr = (int64_t)I * kNB + 16 * w + lk + 4 * v;
+        const double val = -o[v];
+        W[gr * np + gc] = val;
+        if (WT) WT[(int64_t)gc * np + gr] = val;
+        if (W32) {
+          W32[gr * np + gc] = (float)val;
+          WT32[(int64_t)gc * np + gr] = (float)val;
+        }
       }
     }
     __syncthreads();
@@ -592,7 +598,8 @@ void CoarseFactor::factor_async(bool f32out) {
     hipLaunchKernelGGL(k_chol_task, dim3(launch_count[q]), dim3(256), 0, s, A, L, Dinv, np,
                        dtasks + launch_first[q], info);
   const bool f32 = f32out && W32;  // FP32 W, W^T for the mixed V-cycle, else FP64 W^T
-  hipLaunchKernelGGL(k_linv_seq, dim3(np / kStrip), dim3(256), 0, s, L, Dinv, np, dcolinfo,
+  hipLaunchKernelGGL(k_linv_seq, dim3(np / kStrip), dim3(256 * kLinvGroups), 0, s, L, Dinv, np,
+                     dcolinfo,
                      drows, dklist, W, f32 ? nullptr : WT, f32 ? W32 : nullptr,
                      f32 ? WT32 : nullptr);
 }

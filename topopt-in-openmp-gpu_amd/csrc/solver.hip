@@ -124,6 +124,8 @@ void SolverState::release() {
 // library stream
 bool SolverState::setup_main(const double *x) {
   launch_young<double>(*g, x, E64);
+  // the second stream needs only E64
+  if (!T3D_HIP(hipEventRecord(ev_base, stream()))) return false;
   if (mixed) launch_convert<float, double>(g->nelem, E64, E32);
   // the fine Jacobi sweeps form their diagonal from E when they can
   const bool fdiag = fine_sweep_forms_diagonal();
@@ -132,11 +134,11 @@ bool SolverState::setup_main(const double *x) {
   if (mixed)
     for (int l = fdiag ? 1 : 0; l < 2; l++)
       launch_convert<float, double>(g->lv[l].ndof, invD64[l], invD32[l]);
-  return T3D_HIP(hipEventRecord(ev_base, stream()));
+  return true;
 }
 
 // The assembled levels and the coarse factorization, queued on `aux` after
-// the set-up on the library stream (ev_base).
+// the moduli (ev_base).
 bool SolverState::setup_aux() {
   {
     StreamScope on_aux(aux);
