@@ -75,6 +75,7 @@ def load(typing="fp64"):
         "or_change": (c_float, [P, P, c_int64]),
         "or_top3d": (c_int, [c_int, c_int, c_int, c_float, c_float, c_int,
                              c_float, c_int, c_int, P, P, P]),
+        "or_write_vtu": (c_int, [P, P, ctypes.c_char_p, c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -151,6 +152,11 @@ class Grid:
     # ---- operators --------------------------------------------------------
     def _x(self, x):
         return np.ascontiguousarray(x, self.dt)
+
+    def write_vtu(self, x, path, halo=False):
+        """writeDensityVtkFile(WithHalo) of the padded element densities x."""
+        if self.lib.or_write_vtu(self.h, _p(self._x(x)), os.fsencode(path), int(halo)) != 0:
+            raise OSError(f"cannot write {path}")
 
     def matvec0(self, x, v):
         v = np.ascontiguousarray(v, self.ct)

@@ -27,6 +27,11 @@
  *   or_load          total force, node set, count
  *   or_top3d         composition of the above (trajectory fixtures are
  *                    regression data from this file, not independent)
+ *   or_write_vtu     XML parse; point count and integer coordinates; every
+ *                    cell is the unit cube at its element corner; the rigid
+ *                    motions of each cell's node list (in file order) are in
+ *                    the null space of KE_0; offsets, types; densities read
+ *                    back in element order
  *
  * Citations are to files of the reference at the top level of its tree.
  */
@@ -1254,4 +1259,73 @@ int or_top3d(int nelx, int nely, int nelz, float volfrac, float rmin, int nl,
   free(xPhys);
   free(xnew);
   return rc ? rc : loop;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Density snapshot in the ASCII VTK unstructured-grid format:
+ * writeDensityVtkFile (halo = 0) and writeDensityVtkFileWithHalo (halo = 1),
+ * stencil_optimization.c:399-598.  Points sit at integer node coordinates
+ * (x=i, y=j, z=k), x slowest and y fastest.  Each element is a
+ * VTK_HEXAHEDRON (type 12) listing its 8 nodes in the element's local node
+ * order (getEdof_halo, stencil_utility.h:8-26).  One Float32 "density" value
+ * per cell comes from the padded element layout.  halo = 0 writes the
+ * (nel+1)^3 node box from 0 and the real elements 1..nel; halo = 1 the whole
+ * padded grid.  Returns 0, or -1 if the file cannot be opened.             */
+int or_write_vtu(const or_grid *g, const or_dtype *x, const char *path,
+                 int halo) {
+  /* local node q of element (i, j, k) sits at (i, j, k) + corner[q] */
+  static const int corner[8][3] = {{0, 1, 0}, {1, 1, 0}, {1, 0, 0},
+                                   {0, 0, 0}, {0, 1, 1}, {1, 1, 1},
+                                   {1, 0, 1}, {0, 0, 1}}; /* (dx, dy, dz) */
+  int ncell, wx, wy, wz;
+  int64_t ndof;
+  or_padding(g, 0, &ncell, &wx, &wy, &wz, &ndof);
+  const int64_t ewy = wy - 1, ewz = wz - 1;
+  const int px = halo ? wx : g->nelx + 1;
+  const int py = halo ? wy : g->nely + 1;
+  const int pz = halo ? wz : g->nelz + 1;
+  const int cx = px - 1, cy = py - 1, cz = pz - 1;
+  const int e0 = halo ? 0 : 1; /* element index of the first cell */
+  FILE *f = fopen(path, "w");
+  if (!f) return -1;
+  fprintf(f, "<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" "
+             "byte_order=\"LittleEndian\">\n<UnstructuredGrid>\n");
+  fprintf(f, "<Piece NumberOfPoints=\"%i\" NumberOfCells=\"%i\">\n",
+          px * py * pz, cx * cy * cz);
+  fprintf(f, "<Points>\n<DataArray type=\"Float32\" NumberOfComponents=\"3\" "
+             "format=\"ascii\">\n");
+  for (int i = 0; i < px; i++)
+    for (int k = 0; k < pz; k++)
+      for (int j = 0; j < py; j++)
+        fprintf(f, "%e %e %e\n", (double)(float)i, (double)(float)j,
+                (double)(float)k);
+  fprintf(f, "</DataArray>\n</Points>\n<Cells>\n");
+  fprintf(f, "<DataArray type=\"Int32\" Name=\"connectivity\" "
+             "format= \"ascii\">\n");
+  for (int i = 0; i < cx; i++)
+    for (int k = 0; k < cz; k++)
+      for (int j = 0; j < cy; j++)
+        for (int q = 0; q < 8; q++) {
+          const int n = (i + corner[q][0]) * py * pz + (k + corner[q][2]) * py +
+                        (j + corner[q][1]);
+          fprintf(f, q < 7 ? "%d " : "%d\n", n);
+        }
+  fprintf(f, "</DataArray>\n");
+  fprintf(f, "<DataArray type=\"Int32\" Name=\"offsets\" format=\"ascii\">\n");
+  for (int c = 1; c <= cx * cy * cz; c++) fprintf(f, "%d\n", 8 * c);
+  fprintf(f, "</DataArray>\n");
+  fprintf(f, "<DataArray type=\"UInt8\" Name=\"types\" format=\"ascii\">\n");
+  for (int c = 0; c < cx * cy * cz; c++) fprintf(f, "12\n");
+  fprintf(f, "</DataArray>\n</Cells>\n<CellData>\n");
+  fprintf(f, "<DataArray type=\"Float32\" NumberOfComponents=\"1\" "
+             "Name=\"density\" format=\"ascii\">\n");
+  for (int i = 0; i < cx; i++)
+    for (int k = 0; k < cz; k++)
+      for (int j = 0; j < cy; j++)
+        fprintf(f, "%e\n",
+                (double)x[(i + e0) * ewy * ewz + (k + e0) * ewy + (j + e0)]);
+  fprintf(f, "</DataArray>\n</CellData>\n</Piece>\n</UnstructuredGrid>\n"
+             "</VTKFile>\n");
+  fclose(f);
+  return 0;
 }

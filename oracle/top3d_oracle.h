@@ -136,6 +136,8 @@ void or_oc(const or_dtype *x, or_dtype *xnew, const or_dtype *dv,
 int or_top3d(int nelx, int nely, int nelz, float volfrac, float rmin, int nl,
              float cgtol, int iters, int yalign, or_record *records,
              or_dtype *xPhys_out, or_stype *U_out);
+int or_write_vtu(const or_grid *g, const or_dtype *x, const char *path,
+                 int halo);
 
 #ifdef __cplusplus
 }
