@@ -264,6 +264,19 @@ void top3d_set_default_bc_mode(int mode);
 /* the cantilever load of top3dmgcg (stencil_optimization.c:39-95), added
  * into F (host array in the level-0 halo layout) */
 void top3d_cantilever_load(const struct gridContext *gc, CTYPE *F);
+/* the line load of the MATLAB specification (top3dmgcg_reference.m:59):
+ * F_z -= 1 at every node of the edge x = nelx, z = 0, added into F */
+void top3d_line_load(const struct gridContext *gc, CTYPE *F);
+/* supports that setFixedDof_halo (stencil_grid_utility.c:7-65), and so
+ * setupGC, builds from this call on:
+ * TOP3D_SUPPORT_STRIPS  two clamped strips of the x=0 face, y in
+ *                       [0, nely/4] and [nely - nely/4, nely] (as shipped);
+ * TOP3D_SUPPORT_FACE    the whole x=0 face clamped, the variant kept in
+ *                       comments at stencil_grid_utility.c:19-32           */
+#define TOP3D_SUPPORT_STRIPS 0
+#define TOP3D_SUPPORT_FACE 1
+void top3d_set_default_support(int mode);
+int top3d_get_default_support(void);
 
 /* present table (replaces `#pragma omp target enter/exit data/update`) */
 int top3d_map(const void *host, uint64_t bytes);          /* alloc + copy in */

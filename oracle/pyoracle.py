@@ -51,6 +51,8 @@ def load(typing="fp64"):
         "or_create": (P, [c_int, c_int, c_int, c_int, c_int]),
         "or_destroy": (None, [P]),
         "or_set_bcsym": (None, [P, c_int]),
+        "or_set_support": (None, [P, c_int]),
+        "or_line_load": (None, [P, P]),
         "or_padding": (None, [P, c_int, P, P, P, P, P]),
         "or_nfixed": (c_int, [P, c_int]),
         "or_fixed": (None, [P, c_int, P]),
@@ -94,7 +96,8 @@ def _p(a):
 class Grid:
     """One oracle grid (setupGC) with solver buffers created on demand."""
 
-    def __init__(self, nelx, nely, nelz, nl, yalign=1, typing="fp64", bcsym=False):
+    def __init__(self, nelx, nely, nelz, nl, yalign=1, typing="fp64", bcsym=False,
+                 support="strips"):
         self.lib = load(typing)
         self.ct = self.lib._ctype
         self.dt = self.ct
@@ -103,6 +106,9 @@ class Grid:
         if not self.h:
             raise ValueError("invalid oracle grid")
         self.lib.or_set_bcsym(self.h, int(bcsym))
+        if support not in ("strips", "face"):
+            raise ValueError(f"unknown support {support!r}")
+        self.lib.or_set_support(self.h, int(support == "face"))
         self._solver = None
 
     def close(self):
@@ -147,6 +153,11 @@ class Grid:
     def load(self):
         F = np.zeros(self.ndof(0), self.ct)
         self.lib.or_load(self.h, _p(F))
+        return F
+
+    def line_load(self):
+        F = np.zeros(self.ndof(0), self.ct)
+        self.lib.or_line_load(self.h, _p(F))
         return F
 
     # ---- operators --------------------------------------------------------

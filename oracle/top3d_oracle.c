@@ -64,15 +64,27 @@ void or_padding(const or_grid *g, int l, int *ncell, int *wrapx, int *wrapy,
   *ndof = 3 * (int64_t)(*wrapx) * (*wrapy) * (*wrapz);
 }
 
-/* Fixed DOFs of the "new cantilever": setFixedDof_halo,
- * stencil_grid_utility.c:7-65.  Node plane i=1 (the x=0 face), all z, and y
- * in [1, nely_l/4+1] and [nely_l+2-(nely_l/4+1), nely_l+1].                 */
+/* Fixed DOFs, setFixedDof_halo, stencil_grid_utility.c:7-65.
+ * support 0, the "new cantilever" (:34-57): node plane i=1 (the x=0 face),
+ * all z, and y in [1, nely_l/4+1] and [nely_l+2-(nely_l/4+1), nely_l+1].
+ * support 1, the "classic cantilever" left in comments there (:19-32): every
+ * node of plane i=1, k outer and j inner.                                   */
 static void fixed_dofs(or_grid *g, int l) {
   int ncell, wx, wy, wz;
   int64_t ndof;
   or_padding(g, l, &ncell, &wx, &wy, &wz, &ndof);
   const int nelyc = g->nely / ncell, nelzc = g->nelz / ncell;
   const int nyc = nelyc + 1, nzc = nelzc + 1;
+  if (g->support == 1) {
+    g->nfixed[l] = 3 * nyc * nzc;
+    g->fixed[l] = (int64_t *)malloc(sizeof(int64_t) * g->nfixed[l]);
+    int off = 0;
+    for (int64_t k = 1; k <= nzc; k++)
+      for (int64_t j = 1; j <= nyc; j++)
+        for (int c = 0; c < 3; c++)
+          g->fixed[l][off++] = 3 * ((int64_t)wy * wz + wy * k + j) + c;
+    return;
+  }
   const int nodelimit = nelyc / 4 + 1;
   g->nfixed[l] = 3 * nzc * 2 * nodelimit;
   g->fixed[l] = (int64_t *)malloc(sizeof(int64_t) * g->nfixed[l]);
@@ -222,6 +234,14 @@ or_grid *or_create(int nelx, int nely, int nelz, int nl, int yalign) {
  * makes PCG stagnate on strongly heterogeneous densities.                   */
 void or_set_bcsym(or_grid *g, int on) { g->bcsym = on ? 1 : 0; }
 
+void or_set_support(or_grid *g, int mode) {
+  g->support = mode == 1 ? 1 : 0;
+  for (int l = 0; l < g->nl; l++) {
+    free(g->fixed[l]);
+    fixed_dofs(g, l);
+  }
+}
+
 static const or_ctype *masked_input(const or_grid *g, int l, const or_ctype *in,
                                     int64_t ndof, or_ctype **buf) {
   *buf = NULL;
@@ -239,6 +259,19 @@ void or_destroy(or_grid *g) {
     free(g->fixed[l]);
   }
   free(g);
+}
+
+/* Line load of the MATLAB specification (top3dmgcg_reference.m:59, "constant
+ * load"): F_z = -1 at every node of the edge x = nelx, z = 0, i.e. the halo
+ * nodes (nelx+1, j, 1) for j = 1..nely+1.                                   */
+void or_line_load(const or_grid *g, or_ctype *F) {
+  int ncell, wx, wy, wz;
+  int64_t ndof;
+  or_padding(g, 0, &ncell, &wx, &wy, &wz, &ndof);
+  memset(F, 0, sizeof(or_ctype) * ndof);
+  const int64_t i = g->nelx + 1, k = 1;
+  for (int64_t j = 1; j <= g->nely + 1; j++)
+    F[3 * (i * wy * wz + k * wy + j) + 2] = -1;
 }
 
 /* Cantilever load, stencil_optimization.c:39-95: F_z = -1/(4*count) on the 4

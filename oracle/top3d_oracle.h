@@ -52,6 +52,8 @@ typedef struct {
   int64_t *fixed[OR_MAXL];
   int bcsym; /* 0: reference BC rows (identity rows, columns kept);
                 1: symmetric elimination on the matrix-free levels too */
+  int support; /* 0: two clamped strips of the x=0 face (as shipped);
+                  1: the whole x=0 face clamped (the commented variant) */
 } or_grid;
 
 typedef struct {
@@ -79,6 +81,7 @@ int or_sizeof_ctype(void);
 or_grid *or_create(int nelx, int nely, int nelz, int nl, int yalign);
 void or_destroy(or_grid *g);
 void or_set_bcsym(or_grid *g, int on);
+void or_set_support(or_grid *g, int mode);
 void or_padding(const or_grid *g, int l, int *ncell, int *wrapx, int *wrapy,
                 int *wrapz, int64_t *ndof);
 int or_nfixed(const or_grid *g, int l);
@@ -86,6 +89,7 @@ void or_fixed(const or_grid *g, int l, int64_t *out);
 void or_ke(double nu, int l, int npts, double *KE);
 void or_ke_copy(const or_grid *g, int l, double *out);
 void or_load(const or_grid *g, or_ctype *F);
+void or_line_load(const or_grid *g, or_ctype *F);
 
 void or_matvec0(const or_grid *g, const or_dtype *x, const or_ctype *in,
                 or_ctype *out);

@@ -47,6 +47,12 @@ void top3d_cantilever_load(const struct gridContext *gc, CTYPE *F) {
     }
 }
 
+void top3d_line_load(const struct gridContext *gc, CTYPE *F) {
+  const uint64_t wy = gc->wrapy, wz = gc->wrapz;
+  const uint64_t i = gc->nelx + 1, k = 1;
+  for (uint64_t j = 1; j < gc->nely + 2; j++) F[3 * (i * wy * wz + k * wy + j) + 2] -= 1.0;
+}
+
 void top3dmgcg(const uint_fast32_t nelx, const uint_fast32_t nely, const uint_fast32_t nelz,
                const DTYPE volfrac, const DTYPE rmin, const uint_fast32_t nl, const float cgtol,
                const uint_fast32_t cgmax, const int verbose, const int write_result,

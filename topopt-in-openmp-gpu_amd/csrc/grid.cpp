@@ -127,6 +127,8 @@ void *dptr(const void *p, const char *what) {
 // carry one halo node on each side plus one, as in gpu_definitions.c:67-92.
 namespace {
 constexpr int kYAlign = 1;
+// supports that setFixedDof_halo (and so setupGC) builds
+int g_default_support = TOP3D_SUPPORT_STRIPS;
 std::map<const void *, DevGrid *> g_grids;  // keyed by gc->precomputedKE
 std::mutex g_gmu;
 }  // namespace
@@ -303,6 +305,10 @@ using namespace t3d;
 extern "C" {
 
 int top3d_last_error(void) { return g_err; }
+void top3d_set_default_support(int mode) {
+  g_default_support = mode == TOP3D_SUPPORT_FACE ? TOP3D_SUPPORT_FACE : TOP3D_SUPPORT_STRIPS;
+}
+int top3d_get_default_support(void) { return g_default_support; }
 const char *top3d_last_error_string(void) { return g_msg; }
 void top3d_clear_error(void) {
   g_err = TOP3D_OK;
@@ -421,10 +427,25 @@ void computePadding(const struct gridContext *gc, const int l, int *ncell,
   *ndof = (uint_fast32_t)L.ndof;
 }
 
-// stencil_grid_utility.c:7-65 ("new cantilever" supports)
+// stencil_grid_utility.c:7-65.  TOP3D_SUPPORT_STRIPS (the default) clamps
+// the two strips of the x=0 face that the reference builds (:34-57);
+// TOP3D_SUPPORT_FACE clamps every node of that face, the variant the
+// reference keeps in comments (:19-32).
 void setFixedDof_halo(struct gridContext *gc, const int l) {
   Level L;
   compute_level(gc, l, &L);
+  if (g_default_support == TOP3D_SUPPORT_FACE) {
+    gc->fixedDofs[l].n = 3 * (uint64_t)L.ny * L.nz;
+    gc->fixedDofs[l].idx =
+        (uint_fast32_t *)std::malloc(sizeof(uint_fast32_t) * gc->fixedDofs[l].n);
+    int off = 0;
+    for (uint64_t k = 1; k < (uint64_t)L.nz + 1; k++)
+      for (uint64_t j = 1; j < (uint64_t)L.ny + 1; j++)
+        for (int c = 0; c < 3; c++)
+          gc->fixedDofs[l].idx[off++] = 3 * ((uint64_t)L.wy * L.wz + L.wy * k + j) + c;
+    if (dev_grid_quiet(gc)) refresh_fixed(gc, l);
+    return;
+  }
   const int nodelimit = L.nely / 4 + 1;
   gc->fixedDofs[l].n = 3 * L.nz * 2 * nodelimit;
   gc->fixedDofs[l].idx =

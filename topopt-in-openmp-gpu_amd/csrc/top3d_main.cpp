@@ -1,7 +1,8 @@
 // top3d command line (top3d.c:14-88): -x -y -z coarse elements, -l levels,
 // -f volume fraction, -r filter radius, -i design iterations, -v verbose,
-// -w write VTU; plus -p 0|1 (FP64 | mixed precision) and -b 0|1 (reference
-// | symmetric BC rows on the matrix-free levels).
+// -w write VTU; plus -p 0|1 (FP64 | mixed precision), -b 0|1 (reference
+// | symmetric BC rows on the matrix-free levels) and -s 0|1 (two clamped
+// strips of the x=0 face | the whole face clamped).
 #include <getopt.h>
 
 #include <cstdio>
@@ -12,9 +13,9 @@
 int main(int argc, char *argv[]) {
   int nelx_coarse = 12, nely_coarse = 6, nelz_coarse = 6;
   float volfrac = 0.2f, rmin = 1.5f;
-  int iters = 20, nl = 4, verbose = 0, write_result = 0, precision = 0, bc = 0;
+  int iters = 20, nl = 4, verbose = 0, write_result = 0, precision = 0, bc = 0, support = 0;
   int opt;
-  while ((opt = getopt(argc, argv, "x:y:z:r:f:i:l:v:w:p:b:")) != -1) {
+  while ((opt = getopt(argc, argv, "x:y:z:r:f:i:l:v:w:p:b:s:")) != -1) {
     switch (opt) {
       case 'x': nelx_coarse = std::atoi(optarg); break;
       case 'y': nely_coarse = std::atoi(optarg); break;
@@ -27,8 +28,9 @@ int main(int argc, char *argv[]) {
       case 'w': write_result = std::atoi(optarg); break;
       case 'p': precision = std::atoi(optarg); break;
       case 'b': bc = std::atoi(optarg); break;
+      case 's': support = std::atoi(optarg); break;
       default:
-        std::fprintf(stderr, "Usage: %s [-xyzrfilvwpb]\n", argv[0]);
+        std::fprintf(stderr, "Usage: %s [-xyzrfilvwpbs]\n", argv[0]);
         return EXIT_FAILURE;
     }
   }
@@ -46,6 +48,7 @@ int main(int argc, char *argv[]) {
                 volfrac, rmin, iters, verbose, write_result, precision ? "mixed" : "fp64");
   top3d_set_default_precision(precision ? TOP3D_MIXED : TOP3D_FP64);
   top3d_set_default_bc_mode(bc ? TOP3D_BC_SYMMETRIC : TOP3D_BC_REFERENCE);
+  top3d_set_default_support(support ? TOP3D_SUPPORT_FACE : TOP3D_SUPPORT_STRIPS);
   const float cgtol = 1e-5f;
   const int cgmax = 200;
   top3dmgcg(nelx, nely, nelz, volfrac, rmin, nl, cgtol, cgmax, verbose, write_result, iters);

@@ -25,6 +25,8 @@ c_int, c_double, c_float, c_void_p, c_u64 = (ctypes.c_int, ctypes.c_double, ctyp
 
 TOP3D_FP64, TOP3D_MIXED = 0, 1
 TOP3D_BC_REFERENCE, TOP3D_BC_SYMMETRIC = 0, 1
+TOP3D_SUPPORT_STRIPS, TOP3D_SUPPORT_FACE = 0, 1
+SUPPORTS = {"strips": TOP3D_SUPPORT_STRIPS, "face": TOP3D_SUPPORT_FACE}
 ERRORS = {0: "ok", 1: "hip", 2: "argument", 3: "unmapped", 4: "not SPD", 5: "no convergence",
           6: "non-finite"}
 
@@ -133,6 +135,9 @@ SIGNATURES = {
     "top3d_set_default_precision": (None, [c_int]),
     "top3d_set_default_bc_mode": (None, [c_int]),
     "top3d_cantilever_load": (None, [P, P]),
+    "top3d_line_load": (None, [P, P]),
+    "top3d_set_default_support": (None, [c_int]),
+    "top3d_get_default_support": (c_int, []),
     "top3d_map": (c_int, [P, c_u64]),
     "top3d_map_alloc": (c_int, [P, c_u64]),
     "top3d_unmap": (c_int, [P]),
@@ -196,7 +201,8 @@ class Grid:
     (i,j,k) at i*(wrapy-1)*(wrapz-1) + k*(wrapy-1) + j).
     """
 
-    def __init__(self, nelx, nely, nelz, nl, precision="fp64", bc="reference", device="cuda:0"):
+    def __init__(self, nelx, nely, nelz, nl, precision="fp64", bc="reference", device="cuda:0",
+                 support="strips"):
         import torch
         self.torch = torch
         self.lib = library()
@@ -204,7 +210,15 @@ class Grid:
         self.lib.top3d_set_stream(c_void_p(torch.cuda.current_stream(self.device).cuda_stream))
         self.nelx, self.nely, self.nelz, self.nl = nelx, nely, nelz, nl
         self.gc = GridContext()
-        self.lib.setupGC(ctypes.byref(self.gc), nl, nelx, nely, nelz)
+        if support not in SUPPORTS:
+            raise ValueError(f"unknown support {support!r}")
+        self.support = support
+        prev = self.lib.top3d_get_default_support()
+        self.lib.top3d_set_default_support(SUPPORTS[support])
+        try:
+            self.lib.setupGC(ctypes.byref(self.gc), nl, nelx, nely, nelz)
+        finally:
+            self.lib.top3d_set_default_support(prev)
         check("setupGC")
         self.gpu = GpuGrid()
         self.gpu.num_targets = 1
@@ -283,6 +297,11 @@ class Grid:
     def load(self):
         F = np.zeros(self.ndof(0))
         self.lib.top3d_cantilever_load(ctypes.byref(self.gc), F.ctypes.data_as(c_void_p))
+        return self.torch.from_numpy(F).to(self.device)
+
+    def line_load(self):
+        F = np.zeros(self.ndof(0))
+        self.lib.top3d_line_load(ctypes.byref(self.gc), F.ctypes.data_as(c_void_p))
         return self.torch.from_numpy(F).to(self.device)
 
     def ke(self, l):
