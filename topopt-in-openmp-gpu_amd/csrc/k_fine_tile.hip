@@ -177,6 +177,31 @@ static constexpr uint32_t kK1Mask[24] = {
     0xdd3977, 0x650650, 0x650650, 0xffb977, 0xa7b977, 0xdd3977, 0x0,   0xa69820,
     0xfd3f77, 0xa69820, 0xe7bf77, 0xd92900, 0xd92900, 0xe79e70, 0xfd2f50, 0xffb920};
 constexpr int kK1Nnz = 279;
+// The non-zeros take 20 magnitudes: entry t of the row-major non-zero list is
+// kK1Class[t] & 127 of those, negated when bit 7 is set.  The classes hold
+// for every Poisson ratio (k1_classes.py derives them by sampling ratios from
+// 0.05 to 0.49); init_coef checks them against the K1 it computes.
+constexpr int kK1Classes = 20;
+static constexpr uint8_t kK1Class[kK1Nnz] = {
+    0, 1, 129, 2, 130, 3, 130, 4, 3, 2, 132, 5, 1, 0, 1, 3,
+    2, 2, 2, 132, 2, 3, 5, 132, 129, 1, 0, 2, 3, 130, 3, 5,
+    130, 2, 132, 4, 2, 3, 2, 6, 7, 8, 132, 9, 137, 10, 132, 5,
+    11, 12, 13, 141, 130, 2, 3, 7, 6, 4, 5, 11, 136, 132, 137, 10,
+    137, 140, 141, 141, 3, 2, 130, 8, 4, 6, 135, 137, 9, 138, 5, 132,
+    12, 11, 141, 13, 130, 2, 3, 132, 5, 135, 6, 11, 4, 8, 12, 138,
+    9, 137, 13, 141, 9, 137, 14, 142, 13, 141, 137, 9, 137, 9, 142, 14,
+    141, 13, 9, 137, 4, 132, 5, 10, 11, 138, 11, 15, 140, 12, 141, 16,
+    141, 144, 13, 141, 17, 145, 18, 3, 2, 130, 132, 136, 5, 4, 140, 6,
+    7, 137, 10, 137, 11, 141, 141, 2, 3, 2, 5, 132, 132, 8, 12, 7,
+    6, 11, 10, 137, 9, 141, 13, 137, 141, 137, 14, 14, 141, 137, 137, 132,
+    5, 132, 11, 10, 12, 12, 13, 141, 16, 10, 11, 15, 16, 141, 13, 145,
+    18, 145, 137, 141, 137, 14, 14, 141, 137, 137, 5, 132, 4, 12, 140, 11,
+    138, 141, 13, 144, 11, 10, 141, 16, 141, 15, 18, 145, 17, 9, 13, 137,
+    141, 14, 142, 137, 9, 137, 141, 9, 13, 142, 14, 9, 137, 13, 141, 141,
+    137, 9, 17, 141, 137, 145, 137, 18, 19, 146, 18, 141, 13, 13, 9, 137,
+    145, 141, 18, 145, 137, 9, 146, 19, 146, 141, 141, 18, 141, 13, 137, 145,
+    137, 17, 9, 137, 18, 146, 19,
+};
 
 __host__ __device__ constexpr int k1_offset(int row) {
   int o = 0;
@@ -184,26 +209,19 @@ __host__ __device__ constexpr int k1_offset(int row) {
   return o;
 }
 
+// The class values travel as a kernel argument and stay in scalar registers
+// for the whole kernel; every non-zero is one of them, up to a sign that
+// folds into the multiply-add.
 template <typename T>
-struct K1Coef {
-  T v[kK1Nnz];  // non-zeros of K1, row-major over (3 sa + ca, 3 sb + cb)
+struct K1Vals {
+  T v[kK1Classes];
 };
 
-// The kernel reads K1 from constant memory through an offset the compiler
-// cannot see to be zero, so that the 279 values are fetched by scalar loads
-// inside the element loop instead of being held (and spilled) in registers.
-__constant__ float c_k1f[kK1Nnz];
-__constant__ double c_k1d[kK1Nnz];
-
-__device__ __forceinline__ int opaque_zero() {
-  int z;
-  asm volatile("s_mov_b32 %0, 0" : "=s"(z));
-  return z;
-}
-
-template <typename T>
-__device__ __forceinline__ T k1coef(int off, int idx) {
-  if constexpr (sizeof(T) == 8) return c_k1d[off + idx]; else return c_k1f[off + idx];
+template <typename T, int IDX>
+__device__ __forceinline__ T k1coef(const K1Vals<T> &kv) {
+  constexpr uint8_t code = kK1Class[IDX];
+  const T v = kv.v[code & 127];
+  return (code & 128) ? -v : v;
 }
 
 // Host: K1 from the sub-cell matrices KE1 (8 x 576, sub-cell index
@@ -259,55 +277,55 @@ bool level1_hadamard(const double *KE1, double *coef) {
 // of Ehat[sa ^ sb] * sum_cb K1[row][(sb, cb)] H[sb][cb], expanded at compile
 // time over the non-zeros of kK1Mask.
 template <typename T, int ROW, int SB>
-__device__ __forceinline__ void l1_accum(T &acc, int koff, const T (&Eh)[8], const T (&H)[8][3]) {
+__device__ __forceinline__ void l1_accum(T &acc, const K1Vals<T> &kv, const T (&Eh)[8],
+                                         const T (&H)[8][3]) {
   constexpr uint32_t m = (kK1Mask[ROW] >> (3 * SB)) & 7u;
   if constexpr (m != 0) {
     constexpr int base =
         k1_offset(ROW) + __builtin_popcount(kK1Mask[ROW] & ((1u << (3 * SB)) - 1u));
     T part;
     if constexpr ((m & 1u) != 0) {
-      part = k1coef<T>(koff, base) * H[SB][0];
-      if constexpr ((m & 2u) != 0) part += k1coef<T>(koff, base + 1) * H[SB][1];
+      part = k1coef<T, base>(kv) * H[SB][0];
+      if constexpr ((m & 2u) != 0) part += k1coef<T, base + 1>(kv) * H[SB][1];
       if constexpr ((m & 4u) != 0)
-        part += k1coef<T>(koff, base + ((m & 2u) ? 2 : 1)) * H[SB][2];
+        part += k1coef<T, base + ((m & 2u) ? 2 : 1)>(kv) * H[SB][2];
     } else if constexpr ((m & 2u) != 0) {
-      part = k1coef<T>(koff, base) * H[SB][1];
-      if constexpr ((m & 4u) != 0) part += k1coef<T>(koff, base + 1) * H[SB][2];
+      part = k1coef<T, base>(kv) * H[SB][1];
+      if constexpr ((m & 4u) != 0) part += k1coef<T, base + 1>(kv) * H[SB][2];
     } else {
-      part = k1coef<T>(koff, base) * H[SB][2];
+      part = k1coef<T, base>(kv) * H[SB][2];
     }
     acc += Eh[(ROW / 3) ^ SB] * part;
   }
 }
 
 template <typename T, int ROW, int... SB>
-__device__ __forceinline__ T l1_row(int koff, const T (&Eh)[8], const T (&H)[8][3],
+__device__ __forceinline__ T l1_row(const K1Vals<T> &kv, const T (&Eh)[8], const T (&H)[8][3],
                                     std::integer_sequence<int, SB...>) {
   T acc = 0;
-  (l1_accum<T, ROW, SB>(acc, koff, Eh, H), ...);
+  (l1_accum<T, ROW, SB>(acc, kv, Eh, H), ...);
   return acc;
 }
 
 template <typename T, int... ROW>
-__device__ __forceinline__ void l1_rows(T (&g)[8][3], int koff, const T (&Eh)[8],
+__device__ __forceinline__ void l1_rows(T (&g)[8][3], const K1Vals<T> &kv, const T (&Eh)[8],
                                         const T (&H)[8][3], std::integer_sequence<int, ROW...>) {
   ((g[(ROW / 3) ^ (1 << (ROW % 3))][ROW % 3] =
-        l1_row<T, ROW>(koff, Eh, H, std::make_integer_sequence<int, 8>{})),
+        l1_row<T, ROW>(kv, Eh, H, std::make_integer_sequence<int, 8>{})),
    ...);
 }
 
 // g = (T (sum_o E_o KE1[o]) T^T / 64) h for one level-1 element, Eh the
 // transformed moduli
 template <typename T>
-__device__ __forceinline__ void element_core_l1(const T (&Eh)[8], const T (&h)[8][3],
-                                                T (&g)[8][3]) {
-  const int koff = opaque_zero();
+__device__ __forceinline__ void element_core_l1(const K1Vals<T> &kv, const T (&Eh)[8],
+                                                const T (&h)[8][3], T (&g)[8][3]) {
   T H[8][3];  // relabelled: H[sigma][c] = h[sigma ^ (1 << c)][c]
 #pragma unroll
   for (int sg = 0; sg < 8; sg++)
 #pragma unroll
     for (int c = 0; c < 3; c++) H[sg][c] = h[sg ^ (1 << c)][c];
-  l1_rows<T>(g, koff, Eh, H, std::make_integer_sequence<int, 24>{});
+  l1_rows<T>(g, kv, Eh, H, std::make_integer_sequence<int, 24>{});
 }
 
 struct TileArgs {
@@ -322,7 +340,7 @@ struct TileArgs {
 // moduli per element.  MODE: apply / Jacobi sweep / residual.
 template <typename T, int LVL, int MODE, bool BCSYM>
 __device__ __forceinline__ void tile_sweep_body(
-    TileArgs a, std::conditional_t<LVL == 0, Blocks<T>, int> B, const T *__restrict__ E,
+    TileArgs a, std::conditional_t<LVL == 0, Blocks<T>, K1Vals<T>> B, const T *__restrict__ E,
     const T *__restrict__ in, T *__restrict__ out, const T *__restrict__ b,
     const T *__restrict__ invD, T omega, const uint8_t *__restrict__ fix,
     double *__restrict__ dotp) {
@@ -459,7 +477,7 @@ __device__ __forceinline__ void tile_sweep_body(
           Eh[q + 4] = p2[q] - p2[q + 4];
         }
       }
-      element_core_l1<T>(Eh, h, g);
+      element_core_l1<T>(B, Eh, h, g);
     }
     if (i >= i0) {
       // plane i: x- face of slab i plus the pending x+ face of slab i-1,
@@ -585,7 +603,7 @@ __device__ __forceinline__ void tile_sweep_body(
 
 template <typename T, int LVL, int MODE, bool BCSYM>
 __global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_tile_sweep(
-    TileArgs a, std::conditional_t<LVL == 0, Blocks<T>, int> B, const T *__restrict__ E,
+    TileArgs a, std::conditional_t<LVL == 0, Blocks<T>, K1Vals<T>> B, const T *__restrict__ E,
     const T *__restrict__ in, T *__restrict__ out, const T *__restrict__ b,
     const T *__restrict__ invD, T omega, const uint8_t *__restrict__ fix,
     double *__restrict__ dotp) {
@@ -609,8 +627,8 @@ struct HostCoef {
   bool diag0 = false;  // KE_0 has one diagonal value per component
   Blocks<double> b64;
   Blocks<float> b32;
-  K1Coef<double> k64;
-  K1Coef<float> k32;
+  K1Vals<double> k64;  // the class values of K1
+  K1Vals<float> k32;
 };
 static HostCoef g_coef;
 
@@ -642,22 +660,24 @@ static void init_coef() {
   }
   double coef[kK1Nnz];
   g_coef.ok1 = level1_hadamard(KE1.data(), coef);
-  if (g_coef.ok1)
+  if (g_coef.ok1) {
+    // class values from each class's first member (which carries sign +),
+    // then every non-zero checked against its class
+    bool seen[kK1Classes] = {};
+    double scale = 0.0;
     for (int q = 0; q < kK1Nnz; q++) {
-      g_coef.k64.v[q] = coef[q];
-      g_coef.k32.v[q] = (float)coef[q];
+      const int c = kK1Class[q] & 127;
+      if (!seen[c]) g_coef.k64.v[c] = coef[q];
+      seen[c] = true;
+      scale = std::fmax(scale, std::fabs(coef[q]));
     }
-}
-
-// K1 into this device's constant memory (once per device)
-static void upload_k1() {
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return;
-  static uint64_t done = 0;
-  if ((done >> dev) & 1) return;
-  if (T3D_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_k1f), g_coef.k32.v, sizeof(g_coef.k32.v))) &&
-      T3D_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_k1d), g_coef.k64.v, sizeof(g_coef.k64.v))))
-    done |= 1ull << dev;
+    for (int q = 0; q < kK1Nnz; q++) {
+      const double v = g_coef.k64.v[kK1Class[q] & 127];
+      if (std::fabs(coef[q] - ((kK1Class[q] & 128) ? -v : v)) > 1e-12 * scale) g_coef.ok1 = false;
+    }
+    for (int c = 0; c < kK1Classes; c++) g_coef.ok1 = g_coef.ok1 && seen[c];
+    for (int c = 0; c < kK1Classes; c++) g_coef.k32.v[c] = (float)g_coef.k64.v[c];
+  }
 }
 
 bool tile_level1_available() {
@@ -821,12 +841,11 @@ static int launch_tile(const DevGrid &g, int mode, const T *E, const T *in, T *o
   dim3 grid(s.ntx, s.nty * s.ntz);
   const int nwg = s.ntx * s.nty * s.ntz;
   if (dotp && nwg > dot_cap) dotp = nullptr;
-  std::conditional_t<LVL == 0, Blocks<T>, int> B;
+  std::conditional_t<LVL == 0, Blocks<T>, K1Vals<T>> B;
   if constexpr (LVL == 0) {
     if constexpr (sizeof(T) == 8) B = g_coef.b64; else B = g_coef.b32;
   } else {
-    B = 0;
-    upload_k1();
+    if constexpr (sizeof(T) == 8) B = g_coef.k64; else B = g_coef.k32;
   }
 #define T3D_LAUNCH(M, S)                                                                      \
   do {                                                                                       \
