@@ -701,8 +701,11 @@ struct TileShape {
 // Shapes measured on MI355X at 256x128x128 (fine) / 128x64x64 (level 1):
 // for the FP32 fine level 65 x 7 (455 threads: 7 of 8 waves full, 84% of the
 // columns own nodes) ran 49 us in the V-cycle against 55 us for 33 x 8 (264
-// threads, a fifth wave with 8 lanes); wide, short x-marches for level 1.  The x-march length follows from the number of workgroups the device
-// holds at once (`slots`): a launch that overflows them by a few
+// threads, a fifth wave with 8 lanes); wide, short x-marches for level 1:
+// in FP32, 65 x 11 (12 waves, 3 per SIMD, 231 workgroups) ran 17.6 us against
+// 18.6 us for 65 x 12 (13 waves, 198 workgroups).  The x-march length
+// follows from the number of workgroups the device holds at once (`slots`):
+// a launch that overflows them by a few
 // workgroups pays a whole extra round, so NX is chosen to minimise
 // rounds x (NX + 3), the per-workgroup cost in slab steps.
 static TileShape tile_shape(const Level &L, int lvl, int elem_bytes, int &nx_fixed) {
@@ -716,7 +719,7 @@ static TileShape tile_shape(const Level &L, int lvl, int elem_bytes, int &nx_fix
     int ycap, tz, nx;
   };
   static constexpr Rule kRules[2][2] = {{{17, 12, 0}, {65, 7, 0}},  // level 0: f64, f32
-                                        {{65, 6, 8}, {65, 12, 4}}};  // level 1
+                                        {{65, 6, 8}, {65, 11, 4}}};  // level 1
   const Rule r = kRules[lvl == 0 ? 0 : 1][elem_bytes == 4 ? 1 : 0];
   s.TY = std::max(2, std::min(L.nely + 1, r.ycap));
   s.TZ = std::max(2, std::min({L.nelz + 1, r.tz, max_threads / s.TY}));
