@@ -48,7 +48,7 @@ def run_form(top3d_amd, torch, nelx, nely, nelz, nl, precision, warmup, steps):
         print(f"[bench] {precision} warm-up step: solve {r['solve_s']:.4f}s cg {r['cgiter'] + 1}",
               file=sys.stderr, flush=True)
     torch.cuda.synchronize()
-    g.sweep_timing(True)  # HIP events around every fine sweep of the timed solves
+    g.sweep_timing(True)  # HIP events around the runs of fine sweeps of the timed solves
     t0 = time.perf_counter()
     for _ in range(steps):
         recs.append(step())

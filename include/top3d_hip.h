@@ -333,8 +333,10 @@ int top3d_time_kernel(struct SolverDataBuffer *data, int which, int reps,
                       double *avg_ms, double *bytes);
 
 /* Live timing of the fine-level damped-Jacobi sweeps inside the V-cycles of
- * the following solves: HIP events around each sweep launch on the library
- * stream.  enable = 1 clears the accumulators and starts recording, 0 stops.
+ * the following solves: HIP events on the library stream around each run of
+ * consecutive sweeps (the 3 after the first pre-sweep, the 4 post-sweeps),
+ * which execute back to back.  enable = 1 clears the accumulators and starts
+ * recording, 0 stops.
  * Outputs (any may be NULL): mean milliseconds per sweep so far, number of
  * sweeps timed, algorithmic bytes of one sweep (as top3d_time_kernel 0 / 1
  * for the mixed / FP64 form).                                               */

@@ -193,27 +193,34 @@ template <typename T>
 void SolverState::op(int l, int mode, const T *in, T *out, const T *b) {
   const T omega = (T)kOmega;
   const T *invD = invd<T>(l);
-  if (l == 0 && mode == 1 && sweep_timing) {
-    while (sw_ev.size() < sw_used + 2) {
-      hipEvent_t e = nullptr;
-      // timing only: no system-scope fence (cache write-back) at the record,
-      // which would otherwise stall the stream between the timed sweeps
-      if (!T3D_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence))) break;
-      sw_ev.push_back(e);
-    }
-    const bool rec = sw_ev.size() >= sw_used + 2;
-    if (rec) hipEventRecord(sw_ev[sw_used], stream());
-    launch_fine<T>(*g, 3, young_t<T>(), in, out, b, invD, omega, bcsym);
-    if (rec) {
-      hipEventRecord(sw_ev[sw_used + 1], stream());
-      sw_used += 2;
-    }
-  } else if (l == 0)  // Jacobi sweeps form the diagonal from the moduli
+  if (l == 0)  // Jacobi sweeps form the diagonal from the moduli
     launch_fine<T>(*g, mode == 1 ? 3 : mode, young_t<T>(), in, out, b, invD, omega, bcsym);
   else if (l == 1)
     launch_level1<T>(*g, mode, young_t<T>(), in, out, b, invD, omega, bcsym);
   else
     launch_stencil_half<T>(*g, l, mode, stencil_t<T>(l), in, out, b, invD, omega);
+}
+
+// Live sweep timing: an event pair brackets each run of consecutive fine
+// sweeps, which the stream executes back to back.  (A pair around every
+// sweep delays the next launch by 4-8 us per record.)
+bool SolverState::sweep_timing_begin() {
+  if (!sweep_timing) return false;
+  while (sw_ev.size() < sw_used + 2) {
+    hipEvent_t e = nullptr;
+    // timing only: no system-scope fence (cache write-back) at the record
+    if (!T3D_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence))) return false;
+    sw_ev.push_back(e);
+  }
+  return hipEventRecord(sw_ev[sw_used], stream()) == hipSuccess;
+}
+
+void SolverState::sweep_timing_end(bool rec, int nsweeps) {
+  if (!rec) return;
+  if (hipEventRecord(sw_ev[sw_used + 1], stream()) != hipSuccess) return;
+  if (sw_n.size() < sw_used / 2 + 1) sw_n.resize(sw_used / 2 + 1);
+  sw_n[sw_used / 2] = nsweeps;
+  sw_used += 2;
 }
 
 // nswp damped-Jacobi sweeps on level l starting from z = 0 (first sweep
@@ -227,19 +234,23 @@ T *SolverState::smooth_from_zero(int l, const T *b, T *zb, T *tb) {
     launch_fine_jacobi_zero<T, T>(*g, b, young_t<T>(), (T)kOmega, cur);
   else
     launch_jacobi_zero<T, T>(*g, l, b, invd<T>(l), (T)kOmega, cur);
+  const bool rec = l == 0 && nswp > 1 && sweep_timing_begin();
   for (int s = 1; s < nswp; s++) {
     op<T>(l, 1, cur, oth, b);
     std::swap(cur, oth);
   }
+  sweep_timing_end(rec, nswp - 1);
   return cur;  // == zb
 }
 
 template <typename T>
 T *SolverState::smooth(int l, const T *b, T *cur, T *oth) {
+  const bool rec = l == 0 && nswp > 0 && sweep_timing_begin();
   for (int s = 0; s < nswp; s++) {
     op<T>(l, 1, cur, oth, b);
     std::swap(cur, oth);
   }
+  sweep_timing_end(rec, nswp);
   return cur;
 }
 
@@ -280,7 +291,7 @@ void SolverState::collect_sweep_timing() {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, sw_ev[k], sw_ev[k + 1]) == hipSuccess) {
       sw_ms += ms;
-      sw_count++;
+      sw_count += sw_n[k / 2];
     }
   }
   sw_used = 0;

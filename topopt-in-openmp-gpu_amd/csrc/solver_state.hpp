@@ -49,12 +49,16 @@ struct SolverState {
   void join_stencil();
   void join_coarse();
   Top3dSolveStats stats{};
-  // live timing of the fine sweeps (top3d_sweep_timing)
+  // live timing of the fine sweeps (top3d_sweep_timing): one event pair
+  // around each run of consecutive sweeps, sw_n[k] sweeps in pair k
   bool sweep_timing = false;
   std::vector<hipEvent_t> sw_ev;
+  std::vector<int> sw_n;
   size_t sw_used = 0;
   double sw_ms = 0.0;
   int sw_count = 0;
+  bool sweep_timing_begin();
+  void sweep_timing_end(bool rec, int nsweeps);
   void collect_sweep_timing();
 
   bool allocate(DevGrid *grid, int nlev);
