@@ -525,9 +525,12 @@ void launch_jacobi_zero(const DevGrid &g, int l, const TB *b, const T *invD, T o
 // weights 1, 1/2, 1/4, 1/8 by squared distance over the 3x3x3 fine patch.
 template <typename T>
 __global__ void k_restrict(int cnx, int cny, int cnz, int cwy, int cwz, int fwy, int fwz,
-                           const T *__restrict__ fine, T *__restrict__ coarse) {
+                           const T *__restrict__ fine, T *__restrict__ coarse,
+                           const T *__restrict__ invDc, T omega, T *__restrict__ zc) {
   // one thread per interior coarse node; only those are written
-  // (stencil_methods.c:629-686)
+  // (stencil_methods.c:629-686).  With zc, also the coarse level's first
+  // pre-smoothing sweep from zero, zc = omega invDc rc (as k_jacobi_zero; the
+  // other nodes of zc stay zero)
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)cnx * cny * cnz) return;
   const int j = (int)(t % cny) + 1, k = (int)((t / cny) % cnz) + 1,
@@ -550,14 +553,19 @@ __global__ void k_restrict(int cnx, int cny, int cnz, int cwy, int cwz, int fwy,
   coarse[3 * cn] = acc[0];
   coarse[3 * cn + 1] = acc[1];
   coarse[3 * cn + 2] = acc[2];
+  if (zc) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) zc[3 * cn + c] = omega * invDc[3 * cn + c] * (acc[c] - T(0));
+  }
 }
 
 template <typename T>
-void launch_restrict(const DevGrid &g, int l, const T *fine, T *coarse) {
+void launch_restrict(const DevGrid &g, int l, const T *fine, T *coarse, const T *invDc,
+                     T omega, T *zc) {
   const Level &C = g.lv[l + 1], &F = g.lv[l];
   const int64_t nth = C.interior_nodes();
   hipLaunchKernelGGL((k_restrict<T>), dim3((nth + 255) / 256), dim3(256), 0, stream(), C.nx,
-                     C.ny, C.nz, C.wy, C.wz, F.wy, F.wz, fine, coarse);
+                     C.ny, C.nz, C.wy, C.wz, F.wy, F.wz, fine, coarse, invDc, omega, zc);
 }
 
 // Trilinear prolongation level l+1 -> l (stencil_methods.c:521-590);
@@ -764,7 +772,7 @@ void launch_invdiag_stencil(const DevGrid &g, int l, const double *S, T *invD) {
   template void launch_stencil_half<T>(const DevGrid &, int, int, const T *, const T *,  \
                                        T *, const T *, const T *, T);                    \
   template void launch_invdiag_half<T>(const DevGrid &, int, const double *, T *);       \
-  template void launch_restrict<T>(const DevGrid &, int, const T *, T *);                \
+  template void launch_restrict<T>(const DevGrid &, int, const T *, T *, const T *, T, T *); \
   template void launch_prolong_add<T>(const DevGrid &, int, const T *, T *);             \
   template void launch_prolong<T>(const DevGrid &, int, const T *, T *);                 \
   template void launch_invdiag_mf<T>(const DevGrid &, int, const double *, T *);         \

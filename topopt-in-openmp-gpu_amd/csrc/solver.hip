@@ -227,10 +227,12 @@ void SolverState::sweep_timing_end(bool rec, int nsweeps) {
 // without an operator application); ping-pong between z and t.  Returns the
 // buffer holding the result.
 template <typename T>
-T *SolverState::smooth_from_zero(int l, const T *b, T *zb, T *tb) {
+T *SolverState::smooth_from_zero(int l, const T *b, T *zb, T *tb, bool zero_done) {
   T *cur = (nswp % 2 == 1) ? zb : tb;
   T *oth = (cur == zb) ? tb : zb;
-  if (l == 0 && fine_sweep_forms_diagonal())
+  if (zero_done)
+    ;  // written by the restriction that produced b
+  else if (l == 0 && fine_sweep_forms_diagonal())
     launch_fine_jacobi_zero<T, T>(*g, b, young_t<T>(), (T)kOmega, cur);
   else
     launch_jacobi_zero<T, T>(*g, l, b, invd<T>(l), (T)kOmega, cur);
@@ -257,18 +259,24 @@ T *SolverState::smooth(int l, const T *b, T *cur, T *oth) {
 // V-cycle at level l: reads vec<T>(l, kR), leaves the result in vec<T>(l, kZ)
 // (stencil_solvers.c:129-252).
 template <typename T>
-void SolverState::vcycle(int l) {
+void SolverState::vcycle(int l, bool zero_done) {
   T *r_ = vec<T>(l, kR), *z_ = vec<T>(l, kZ), *d_ = vec<T>(l, kD), *t_ = tmp<T>(l);
   if (l >= 2) join_stencil();
-  T *zc = smooth_from_zero<T>(l, r_, z_, t_);
+  T *zc = smooth_from_zero<T>(l, r_, z_, t_, zero_done);
   T *other = (zc == z_) ? t_ : z_;
   op<T>(l, 2, zc, d_, r_);  // d = r - K z
-  launch_restrict<T>(*g, l, d_, vec<T>(l + 1, kR));
   if (nl == l + 2) {
+    launch_restrict<T>(*g, l, d_, vec<T>(l + 1, kR));
     join_coarse();
     launch_coarse_apply<T>(*g, l + 1, coarse, vec<T>(l + 1, kR), vec<T>(l + 1, kZ));
   } else {
-    vcycle<T>(l + 1);
+    // the restriction also writes level l+1's first sweep from zero into the
+    // buffer smooth_from_zero starts in; level l+1's inverse diagonal is
+    // ready (library stream for l+1 = 1, the stencil event for l+1 >= 2)
+    if (l + 1 >= 2) join_stencil();
+    T *first = (nswp % 2 == 1) ? vec<T>(l + 1, kZ) : tmp<T>(l + 1);
+    launch_restrict<T>(*g, l, d_, vec<T>(l + 1, kR), invd<T>(l + 1), (T)kOmega, first);
+    vcycle<T>(l + 1, true);
   }
   launch_prolong_add<T>(*g, l, vec<T>(l + 1, kZ), zc);  // z += P zc
   T *res = smooth<T>(l, r_, zc, other);
@@ -395,8 +403,8 @@ int SolverState::solve(const double *x, const double *b, double *u, double tol, 
   return status;
 }
 
-template void SolverState::vcycle<double>(int);
-template void SolverState::vcycle<float>(int);
+template void SolverState::vcycle<double>(int, bool);
+template void SolverState::vcycle<float>(int, bool);
 template void SolverState::op<double>(int, int, const double *, double *, const double *);
 template void SolverState::op<float>(int, int, const float *, float *, const float *);
 

@@ -70,8 +70,10 @@ struct SolverState {
             float *final_res);
 
   template <typename T> void op(int l, int mode, const T *in, T *out, const T *b);
-  template <typename T> void vcycle(int l);
-  template <typename T> T *smooth_from_zero(int l, const T *b, T *zb, T *tb);
+  // zero_done: the first sweep from zero of level l is already in its buffer
+  template <typename T> void vcycle(int l, bool zero_done = false);
+  template <typename T> T *smooth_from_zero(int l, const T *b, T *zb, T *tb,
+                                            bool zero_done = false);
   template <typename T> T *smooth(int l, const T *b, T *cur, T *oth);
 
   template <typename T> T *vec(int l, int which) {

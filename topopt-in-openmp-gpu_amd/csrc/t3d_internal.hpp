@@ -138,7 +138,8 @@ template <typename TB, typename T>
 void launch_jacobi_zero(const DevGrid &g, int l, const TB *b, const T *invD,
                         T omega, T *out);
 template <typename T>
-void launch_restrict(const DevGrid &g, int l, const T *fine, T *coarse);
+void launch_restrict(const DevGrid &g, int l, const T *fine, T *coarse,
+                     const T *invDc = nullptr, T omega = T(0), T *zc = nullptr);
 template <typename T>
 void launch_prolong_add(const DevGrid &g, int l, const T *coarse, T *fine);
 template <typename T>
