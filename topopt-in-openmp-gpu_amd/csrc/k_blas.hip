@@ -188,6 +188,20 @@ __global__ __launch_bounds__(kRedBlock) void k_axpy_fused(
   }
 }
 
+// r.r from the partials; also into pinned host memory when host_rr is given
+// (replacing a device-to-host copy)
+__global__ __launch_bounds__(kRedBlock) void k_sum_rr(const double *__restrict__ partials, int n,
+                                                      double *__restrict__ out,
+                                                      double *host_rr) {
+  double s = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += partials[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) {
+    *out = s;
+    if (host_rr) __hip_atomic_store(host_rr, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
 template <typename TZ>
 void launch_dot_rz(const double *r, const TZ *z, int64_t n, double *partials) {
   hipLaunchKernelGGL((k_dot_rz<TZ>), dim3(kRedGrid), dim3(kRedBlock), 0, stream(), r, z, n,
@@ -202,11 +216,11 @@ void launch_pupdate_fused(int64_t n, const double *partials, double *scal, doubl
 }
 void launch_axpy_fused(int64_t n, const double *pq_part, int npart, double *scal, double *u,
                        double *r, const double *p, const double *q, float *r32,
-                       double *rr_part, double *rr) {
+                       double *rr_part, double *rr, double *host_rr) {
   hipLaunchKernelGGL(k_axpy_fused, dim3(kRedGrid), dim3(kRedBlock), 0, stream(), n, pq_part,
                      npart, scal, u, r, p, q, r32, rr_part);
-  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kRedBlock), 0, stream(), rr_part, kRedGrid,
-                     rr);
+  hipLaunchKernelGGL(k_sum_rr, dim3(1), dim3(kRedBlock), 0, stream(), rr_part, kRedGrid, rr,
+                     host_rr);
 }
 template void launch_dot_rz<double>(const double *, const double *, int64_t, double *);
 template void launch_dot_rz<float>(const double *, const float *, int64_t, double *);

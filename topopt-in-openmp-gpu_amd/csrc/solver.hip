@@ -76,7 +76,8 @@ bool SolverState::allocate(DevGrid *grid, int nlev) {
   scal = alloc_t<double>(16);
   partials = alloc_t<double>(1024 + 16);
   cgpart = alloc_t<double>(kPartRZ + kPartPQ + kPartRR);
-  if (!T3D_HIP(hipHostMalloc((void **)&hscal, sizeof(double) * 16, hipHostMallocDefault)))
+  if (!T3D_HIP(hipHostMalloc((void **)&hscal, sizeof(double) * 16, hipHostMallocDefault)) ||
+      !T3D_HIP(hipHostGetDevicePointer((void **)&hscal_dev, hscal, 0)))
     return false;
   if (!T3D_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming))) return false;
   if (!T3D_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking)) ||
@@ -358,9 +359,9 @@ int SolverState::solve(const double *x, const double *b, double *u, double tol, 
       npq = kPartRZ;
     }
     // u += alpha p, r -= alpha q (and its FP32 copy), r.r
+    // (r.r also lands in hscal[3], written by the kernel)
     launch_axpy_fused(n0, part_pq, npq, scal, u, r, p, q, mixed ? v32[0][kR] : nullptr,
-                      part_rr, scal + 3);
-    hipMemcpyAsync(hscal + 3, scal + 3, sizeof(double), hipMemcpyDeviceToHost, stream());
+                      part_rr, scal + 3, hscal_dev + 3);
     hipEventRecord(ev, stream());
     // queue the next V-cycle while the residual norm travels to the host; if
     // this iteration converged its result is simply not used

@@ -38,6 +38,7 @@ struct SolverState {
   static constexpr int kPartRZ = 1024, kPartPQ = 1 << 16, kPartRR = 1024;
   double *scal = nullptr, *partials = nullptr, *hscal = nullptr;
   double *cgpart = nullptr;  // partial sums of r.z, p.q, r.r
+  double *hscal_dev = nullptr;  // device address of hscal
   hipEvent_t ev = nullptr;
   // The assembled levels and the coarse factorization are built on `aux`,
   // concurrently with the start of the solve on the library stream, which
