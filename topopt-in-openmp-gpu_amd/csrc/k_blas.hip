@@ -143,36 +143,56 @@ __global__ __launch_bounds__(kRedBlock) void k_dot_rz(const double *__restrict__
 }
 
 // rho = sum(partials); p = z (first) or p = (rho/rho_old) p + z
-// (stencil_solvers.c:350-362); block 0 stores s[0] = rho
+// (stencil_solvers.c:350-362); block 0 stores s[0] = rho.  After the first
+// iteration it also applies the previous iteration's u += alpha p
+// (stencil_solvers.c:366-371) with the p it replaces, alpha = s[1] / s[2] as
+// the CG update left them, so that p is read once for both.
 template <typename TZ>
 __global__ __launch_bounds__(kRedBlock) void k_pupdate_fused(
     int64_t n, const double *__restrict__ partials, int npart, double *__restrict__ s,
-    double *__restrict__ p, const TZ *__restrict__ z, int first) {
+    double *__restrict__ p, const TZ *__restrict__ z, int first, double *__restrict__ u) {
   const double rho = sum_partials_all(partials, npart);
   if (blockIdx.x == 0 && threadIdx.x == 0) s[0] = rho;
   const double beta = first ? 0.0 : rho / s[1];
+  const double alpha = first ? 0.0 : s[1] / s[2];
 #pragma unroll 4
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const double zi = (double)z[i];
-    p[i] = first ? zi : beta * p[i] + zi;
+    if (first) {
+      p[i] = zi;
+    } else {
+      const double pi = p[i];
+      u[i] += alpha * pi;
+      p[i] = beta * pi + zi;
+    }
   }
 }
 
-// pq = sum(partials); alpha = rho / pq; u += alpha p; r -= alpha q
-// (stencil_solvers.c:366-377); r32 = (float) r when given; r.r partials.
-// Block 0 stores rho_old = rho and pq.
+// u += alpha p with alpha = s[1] / s[2]: the last CG update's step, which
+// k_pupdate_fused would have applied in the next iteration
+__global__ void k_u_update(int64_t n, const double *__restrict__ s, double *__restrict__ u,
+                           const double *__restrict__ p) {
+  const double alpha = s[1] / s[2];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    u[i] += alpha * p[i];
+}
+
+// pq = sum(partials); alpha = rho / pq; r -= alpha q (stencil_solvers.c:
+// 366-377); r32 = (float) r when given; r.r partials.  Block 0 stores
+// rho_old = rho and pq; u += alpha p is left to k_pupdate_fused (or, after
+// the last iteration, k_u_update).
 __global__ __launch_bounds__(kRedBlock) void k_axpy_fused(
     int64_t n, const double *__restrict__ pq_part, int npart, double *__restrict__ s,
-    double *__restrict__ u, double *__restrict__ r, const double *__restrict__ p,
-    const double *__restrict__ q, float *__restrict__ r32, double *__restrict__ rr_part) {
+    double *__restrict__ r, const double *__restrict__ q, float *__restrict__ r32,
+    double *__restrict__ rr_part) {
   const double pq = sum_partials_all(pq_part, npart);
   const double rho = s[0];
   const double alpha = rho / pq;
   double acc = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    u[i] += alpha * p[i];
     const double ri = r[i] - alpha * q[i];
     r[i] = ri;
     if (r32) r32[i] = (float)ri;
@@ -209,25 +229,29 @@ void launch_dot_rz(const double *r, const TZ *z, int64_t n, double *partials) {
 }
 template <typename TZ>
 void launch_pupdate_fused(int64_t n, const double *partials, double *scal, double *p,
-                          const TZ *z, int first) {
+                          const TZ *z, int first, double *u) {
   // 2048 blocks: 8 waves per SIMD on 256 CUs for this streaming update
   hipLaunchKernelGGL((k_pupdate_fused<TZ>), dim3(2 * kRedGrid), dim3(kRedBlock), 0, stream(), n,
-                     partials, kRedGrid, scal, p, z, first);
+                     partials, kRedGrid, scal, p, z, first, u);
 }
-void launch_axpy_fused(int64_t n, const double *pq_part, int npart, double *scal, double *u,
-                       double *r, const double *p, const double *q, float *r32,
-                       double *rr_part, double *rr, double *host_rr) {
+void launch_u_update(int64_t n, const double *scal, double *u, const double *p) {
+  hipLaunchKernelGGL(k_u_update, dim3(2 * kRedGrid), dim3(kRedBlock), 0, stream(), n, scal, u,
+                     p);
+}
+void launch_axpy_fused(int64_t n, const double *pq_part, int npart, double *scal, double *r,
+                       const double *q, float *r32, double *rr_part, double *rr,
+                       double *host_rr) {
   hipLaunchKernelGGL(k_axpy_fused, dim3(kRedGrid), dim3(kRedBlock), 0, stream(), n, pq_part,
-                     npart, scal, u, r, p, q, r32, rr_part);
+                     npart, scal, r, q, r32, rr_part);
   hipLaunchKernelGGL(k_sum_rr, dim3(1), dim3(kRedBlock), 0, stream(), rr_part, kRedGrid, rr,
                      host_rr);
 }
 template void launch_dot_rz<double>(const double *, const double *, int64_t, double *);
 template void launch_dot_rz<float>(const double *, const float *, int64_t, double *);
 template void launch_pupdate_fused<double>(int64_t, const double *, double *, double *,
-                                           const double *, int);
+                                           const double *, int, double *);
 template void launch_pupdate_fused<float>(int64_t, const double *, double *, double *,
-                                          const float *, int);
+                                          const float *, int, double *);
 
 template <typename TO, typename TI>
 __global__ void k_convert(int64_t n, const TI *__restrict__ in, TO *__restrict__ out) {

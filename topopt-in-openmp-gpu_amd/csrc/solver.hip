@@ -347,10 +347,10 @@ int SolverState::solve(const double *x, const double *b, double *u, double tol, 
     // rho = r.z; p = z + (rho / rho_old) p
     if (mixed) {
       launch_dot_rz<float>(r, v32[0][kZ], n0, part_rz);
-      launch_pupdate_fused<float>(n0, part_rz, scal, p, v32[0][kZ], iter == 0);
+      launch_pupdate_fused<float>(n0, part_rz, scal, p, v32[0][kZ], iter == 0, u);
     } else {
       launch_dot_rz<double>(r, z, n0, part_rz);
-      launch_pupdate_fused<double>(n0, part_rz, scal, p, z, iter == 0);
+      launch_pupdate_fused<double>(n0, part_rz, scal, p, z, iter == 0, u);
     }
     // q = K p with the partial sums of p.q
     int npq = launch_fine_apply_dot(*g, E64, p, q, bcsym, part_pq, kPartPQ);
@@ -358,10 +358,10 @@ int SolverState::solve(const double *x, const double *b, double *u, double tol, 
       launch_dot_rz<double>(p, q, n0, part_pq);
       npq = kPartRZ;
     }
-    // u += alpha p, r -= alpha q (and its FP32 copy), r.r
-    // (r.r also lands in hscal[3], written by the kernel)
-    launch_axpy_fused(n0, part_pq, npq, scal, u, r, p, q, mixed ? v32[0][kR] : nullptr,
-                      part_rr, scal + 3, hscal_dev + 3);
+    // r -= alpha q (and its FP32 copy), r.r (also into hscal[3]); u += alpha p
+    // follows with the next p update
+    launch_axpy_fused(n0, part_pq, npq, scal, r, q, mixed ? v32[0][kR] : nullptr, part_rr,
+                      scal + 3, hscal_dev + 3);
     hipEventRecord(ev, stream());
     // queue the next V-cycle while the residual norm travels to the host; if
     // this iteration converged its result is simply not used
@@ -384,6 +384,7 @@ int SolverState::solve(const double *x, const double *b, double *u, double tol, 
     }
     if (relres < tol) break;
   }
+  if (max_iter > 0) launch_u_update(n0, scal, u, p);  // the last iteration's u += alpha p
   join_coarse();  // the work on `aux` has finished before the solve returns
   hipStreamWaitEvent(stream(), ev_done, 0);
   if (!T3D_HIP(hipStreamSynchronize(stream()))) status = TOP3D_ERR_HIP;

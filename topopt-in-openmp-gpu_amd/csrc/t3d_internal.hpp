@@ -247,13 +247,16 @@ void launch_pupdate(int64_t n, const double *scal, double *p, const double *z,
 // partials for launch_dot_rz and the r.r partials)
 template <typename TZ>
 void launch_dot_rz(const double *r, const TZ *z, int64_t n, double *partials);
+// p update; from the second iteration on also the previous u += alpha p
 template <typename TZ>
 void launch_pupdate_fused(int64_t n, const double *partials, double *scal, double *p,
-                          const TZ *z, int first);
-// r.r goes to *rr and, when given, to the pinned host word host_rr
-void launch_axpy_fused(int64_t n, const double *pq_part, int npart, double *scal, double *u,
-                       double *r, const double *p, const double *q, float *r32,
-                       double *rr_part, double *rr, double *host_rr);
+                          const TZ *z, int first, double *u);
+// u += alpha p of the last CG update (alpha from scal)
+void launch_u_update(int64_t n, const double *scal, double *u, const double *p);
+// r -= alpha q; r.r goes to *rr and, when given, to the pinned host word host_rr
+void launch_axpy_fused(int64_t n, const double *pq_part, int npart, double *scal, double *r,
+                       const double *q, float *r32, double *rr_part, double *rr,
+                       double *host_rr);
 // FP64 fine apply q = K p with per-workgroup partials of p.q; returns their
 // count (0 if `cap` is too small)
 int launch_fine_apply_dot(const DevGrid &g, const double *E, const double *p, double *q,
