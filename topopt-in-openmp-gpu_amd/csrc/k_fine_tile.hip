@@ -613,7 +613,7 @@ __global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_tile_sweep(
 // fine level in FP32: workgroups of at most 512 threads, which lets the
 // compiler keep the kernel under 72 VGPRs (7 waves per SIMD)
 template <int MODE, bool BCSYM>
-__global__ __launch_bounds__(512) void k_tile_sweep_f32(
+__global__ __launch_bounds__(1024) void k_tile_sweep_f32(
     TileArgs a, Blocks<float> B, const float *__restrict__ E, const float *__restrict__ in,
     float *__restrict__ out, const float *__restrict__ b, const float *__restrict__ invD,
     float omega, const uint8_t *__restrict__ fix, double *__restrict__ dotp) {
@@ -695,13 +695,15 @@ struct TileShape {
 };
 
 // Tile of TY x TZ element columns marching over NX node planes.  LDS holds
-// two exchange buffers of 9 or 10 values per thread (under 60 KiB).  Tuning
+// two exchange buffers of 9 or 10 values per thread (under 80 KiB).  Tuning
 // override: TOP3D_TILE_L<level>=TY,TZ,NX (NX = 0: from the occupancy).
 //
 // Shapes measured on MI355X at 256x128x128 (fine) / 128x64x64 (level 1):
-// for the FP32 fine level 65 x 7 (455 threads: 7 of 8 waves full, 84% of the
-// columns own nodes) ran 49 us in the V-cycle against 55 us for 33 x 8 (264
-// threads, a fifth wave with 8 lanes); wide, short x-marches for level 1:
+// for the FP32 fine level 65 x 11 (715 threads in 12 waves, two workgroups
+// per CU; z tiles of 11 element rows own 10) ran 44.7 us in the V-cycle
+// against 47.3 us for 65 x 7, 46.1 us for 65 x 9, 48.6 us for 33 x 22 and
+// 55.6 us for 65 x 15 (one workgroup per CU); wide, short x-marches for
+// level 1:
 // in FP32, 65 x 11 (12 waves, 3 per SIMD, 231 workgroups) ran 17.6 us against
 // 18.6 us for 65 x 12 (13 waves, 198 workgroups).  The x-march length
 // follows from the number of workgroups the device holds at once (`slots`):
@@ -712,13 +714,13 @@ static TileShape tile_shape(const Level &L, int lvl, int elem_bytes, int &nx_fix
   TileShape s{};
   // LDS values per thread: 2 buffers of 9 (10 on the fine level, whose
   // diagonal-forming sweep also exchanges a modulus)
-  const int lds_threads = 60 * 1024 / ((lvl == 0 ? 20 : 18) * elem_bytes);
-  // launch bounds: 512 threads except level 1 in FP32 (1024)
-  const int max_threads = std::min((elem_bytes == 4 && lvl == 1) ? 1024 : 512, lds_threads);
+  const int lds_threads = 80 * 1024 / ((lvl == 0 ? 20 : 18) * elem_bytes);
+  // launch bounds: 1024 threads in FP32, 512 in FP64
+  const int max_threads = std::min(elem_bytes == 4 ? 1024 : 512, lds_threads);
   struct Rule {
     int ycap, tz, nx;
   };
-  static constexpr Rule kRules[2][2] = {{{17, 12, 0}, {65, 7, 0}},  // level 0: f64, f32
+  static constexpr Rule kRules[2][2] = {{{17, 12, 0}, {65, 11, 0}},  // level 0: f64, f32
                                         {{65, 6, 8}, {65, 11, 4}}};  // level 1
   const Rule r = kRules[lvl == 0 ? 0 : 1][elem_bytes == 4 ? 1 : 0];
   s.TY = std::max(2, std::min(L.nely + 1, r.ycap));
