@@ -406,14 +406,20 @@ __device__ __forceinline__ T ld_at(const T *base, uint32_t byte_off) {
   return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
 }
 
+// The device deals workgroups to its 8 XCDs in turn.  Workgroup w works on
+// (node block, row) pair v = (w % 8) * per + w / 8 of the 3 * nblk pairs,
+// so each XCD sweeps one contiguous eighth of the stencil, the same eighth
+// in every sweep, and can keep it in its own L2 across sweeps.
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_stencil_half(
     int nx, int ny, int nz, int wy, int wz, const T *__restrict__ Sh,
     const T *__restrict__ in, T *__restrict__ out, const T *__restrict__ b,
-    const T *__restrict__ invD, T omega) {
+    const T *__restrict__ invD, T omega, uint32_t nblk, uint32_t per) {
+  const uint32_t v = (blockIdx.x % 8u) * per + blockIdx.x / 8u;
+  if (v >= 3u * nblk) return;
   const uint32_t nint = (uint32_t)nx * ny * nz;
-  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-  const int r = blockIdx.y;
+  const uint32_t n = (v / 3u) * blockDim.x + threadIdx.x;
+  const int r = (int)(v % 3u);
   if (n >= nint) return;
   const uint32_t nyz = (uint32_t)ny * nz;
   const int i = (int)(n / nyz), k = (int)((n - i * nyz) / ny), j = (int)(n - i * nyz - k * ny);
@@ -469,16 +475,17 @@ void launch_stencil_half(const DevGrid &g, int l, int mode, const T *Sh, const T
     set_error(TOP3D_ERR_ARG, "level %d too large for 32-bit stencil offsets", l);
     return;
   }
-  dim3 bs(256), gs((unsigned)((nint + 255) / 256), 3);
+  const uint32_t nblk = (uint32_t)((nint + 255) / 256), per = (3 * nblk + 7) / 8;
+  dim3 bs(256), gs(8 * per);
   if (mode == kApply)
     hipLaunchKernelGGL((k_stencil_half<T, kApply>), gs, bs, 0, stream(), L.nx, L.ny, L.nz,
-                       L.wy, L.wz, Sh, in, out, b, invD, omega);
+                       L.wy, L.wz, Sh, in, out, b, invD, omega, nblk, per);
   else if (mode == kJacobi)
     hipLaunchKernelGGL((k_stencil_half<T, kJacobi>), gs, bs, 0, stream(), L.nx, L.ny, L.nz,
-                       L.wy, L.wz, Sh, in, out, b, invD, omega);
+                       L.wy, L.wz, Sh, in, out, b, invD, omega, nblk, per);
   else
     hipLaunchKernelGGL((k_stencil_half<T, kResidual>), gs, bs, 0, stream(), L.nx, L.ny,
-                       L.nz, L.wy, L.wz, Sh, in, out, b, invD, omega);
+                       L.nz, L.wy, L.wz, Sh, in, out, b, invD, omega, nblk, per);
 }
 
 template <typename T>
