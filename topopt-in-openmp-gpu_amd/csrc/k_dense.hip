@@ -582,8 +582,9 @@ int CoarseFactor::factor(bool f32out) {
 int CoarseFactor::check_info() {
   int hinfo = 0;
   hipStream_t s = stream();
-  hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s);
-  if (!T3D_HIP(hipStreamSynchronize(s))) return TOP3D_ERR_HIP;
+  if (!T3D_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s)) ||
+      !T3D_HIP(hipStreamSynchronize(s)))
+    return TOP3D_ERR_HIP;
   if (hinfo) {
     set_error(TOP3D_ERR_NOT_SPD, "coarse Cholesky: non-positive pivot");
     return TOP3D_ERR_NOT_SPD;
@@ -593,7 +594,7 @@ int CoarseFactor::check_info() {
 
 void CoarseFactor::factor_async(bool f32out) {
   hipStream_t s = stream();
-  hipMemsetAsync(info, 0, sizeof(int), s);
+  if (!T3D_HIP(hipMemsetAsync(info, 0, sizeof(int), s))) return;
   for (size_t q = 0; q < launch_first.size(); q++)
     hipLaunchKernelGGL(k_chol_task, dim3(launch_count[q]), dim3(256), 0, s, A, L, Dinv, np,
                        dtasks + launch_first[q], info);
@@ -638,6 +639,8 @@ __global__ __launch_bounds__(256) void k_trmv_lower(const T *__restrict__ Wm, in
   const int row = blockIdx.x;
   const T *a = Wm + (int64_t)row * n;
   double s = 0.0;
+  // (unrolled so that several loads are in flight; the sum order is unchanged)
+#pragma unroll 8
   for (int c = threadIdx.x; c <= row; c += 256) s += (double)a[c] * (double)x[c];
   s = row_block_sum(s);
   if (threadIdx.x == 0) y[row] = (T)s;
@@ -652,6 +655,7 @@ __global__ __launch_bounds__(256) void k_trmv_upper_scatter(int nx, int ny, int 
   const int row = blockIdx.x;
   const T *a = WTm + (int64_t)row * n;
   double s = 0.0;
+#pragma unroll 8
   for (int c = row + threadIdx.x; c < n; c += 256) s += (double)a[c] * (double)y[c];
   s = row_block_sum(s);
   const int d = ipos[row];
@@ -697,9 +701,10 @@ extern "C" int top3d_dense_spd_inverse(double *A_dev, int n, int bw) {
     return top3d_last_error() ? top3d_last_error() : TOP3D_ERR_HIP;
   }
   cf.begin_assembly();
-  hipMemcpy2DAsync(cf.A, sizeof(double) * cf.np, A_dev, sizeof(double) * n, sizeof(double) * n,
-                   n, hipMemcpyDeviceToDevice, stream());
-  int rc = cf.factor();
+  int rc = T3D_HIP(hipMemcpy2DAsync(cf.A, sizeof(double) * cf.np, A_dev, sizeof(double) * n,
+                                    sizeof(double) * n, n, hipMemcpyDeviceToDevice, stream()))
+               ? cf.factor()
+               : TOP3D_ERR_HIP;
   if (rc == TOP3D_OK) {
     const int nblk = (n + kNB - 1) / kNB;
     hipLaunchKernelGGL(k_gemm_wtw, dim3(nblk * nblk), dim3(256), 0, stream(), cf.WT, cf.W, n,
