@@ -40,21 +40,27 @@ __global__ __launch_bounds__(kRedBlock) void k_dot_partial(const T *__restrict__
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
-// stage 2: one block sums the partials in a fixed order
-__global__ __launch_bounds__(kRedBlock) void k_sum_partials(const double *__restrict__ partials,
-                                                            int n, double *__restrict__ out) {
+// stage 2: one block sums the partials in a fixed order (r.r, b.b), also into
+// pinned host memory when host_rr is given (in place of a device-to-host copy)
+__global__ __launch_bounds__(kRedBlock) void k_sum_rr(const double *__restrict__ partials, int n,
+                                                      double *__restrict__ out,
+                                                      double *host_rr) {
   double s = 0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) s += partials[i];
   s = block_sum(s);
-  if (threadIdx.x == 0) *out = s;
+  if (threadIdx.x == 0) {
+    *out = s;
+    if (host_rr) __hip_atomic_store(host_rr, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
 }
 
 template <typename T>
-void launch_dot(const T *a, const T *b, int64_t n, double *partials, double *result) {
+void launch_dot(const T *a, const T *b, int64_t n, double *partials, double *result,
+                double *host) {
   hipLaunchKernelGGL((k_dot_partial<T>), dim3(kRedGrid), dim3(kRedBlock), 0, stream(), a,
                      b, n, partials);
-  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kRedBlock), 0, stream(), partials,
-                     kRedGrid, result);
+  hipLaunchKernelGGL(k_sum_rr, dim3(1), dim3(kRedBlock), 0, stream(), partials, kRedGrid,
+                     result, host);
 }
 
 // Device-resident CG scalars (layout shared with the solver):
@@ -208,20 +214,6 @@ __global__ __launch_bounds__(kRedBlock) void k_axpy_fused(
   }
 }
 
-// r.r from the partials; also into pinned host memory when host_rr is given
-// (replacing a device-to-host copy)
-__global__ __launch_bounds__(kRedBlock) void k_sum_rr(const double *__restrict__ partials, int n,
-                                                      double *__restrict__ out,
-                                                      double *host_rr) {
-  double s = 0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) s += partials[i];
-  s = block_sum(s);
-  if (threadIdx.x == 0) {
-    *out = s;
-    if (host_rr) __hip_atomic_store(host_rr, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-  }
-}
-
 template <typename TZ>
 void launch_dot_rz(const double *r, const TZ *z, int64_t n, double *partials) {
   hipLaunchKernelGGL((k_dot_rz<TZ>), dim3(kRedGrid), dim3(kRedBlock), 0, stream(), r, z, n,
@@ -314,8 +306,10 @@ void launch_copy_fixed(const DevGrid &g, int l, const double *in, double *out) {
                      g.fixed[l], in, out);
 }
 
-template void launch_dot<double>(const double *, const double *, int64_t, double *, double *);
-template void launch_dot<float>(const float *, const float *, int64_t, double *, double *);
+template void launch_dot<double>(const double *, const double *, int64_t, double *, double *,
+                                 double *);
+template void launch_dot<float>(const float *, const float *, int64_t, double *, double *,
+                                double *);
 template void launch_convert<float, double>(int64_t, const double *, float *);
 template void launch_convert<double, float>(int64_t, const float *, double *);
 template void launch_convert<double, double>(int64_t, const double *, double *);

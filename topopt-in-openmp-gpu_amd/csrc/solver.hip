@@ -321,10 +321,9 @@ int SolverState::solve(const double *x, const double *b, double *u, double tol, 
     return stats.status;
   }
   launch_fine<double>(*g, 2, E64, z, r, b, nullptr, 0.0, bcsym);  // r = b - K z
-  launch_dot<double>(b, b, n0, partials, scal + 4);
-  if (mixed) launch_convert<float, double>(n0, r, v32[0][kR]);
   hscal[4] = 0.0;
-  hipMemcpyAsync(hscal + 4, scal + 4, sizeof(double), hipMemcpyDeviceToHost, stream());
+  launch_dot<double>(b, b, n0, partials, scal + 4, hscal_dev + 4);  // b.b, also into hscal[4]
+  if (mixed) launch_convert<float, double>(n0, r, v32[0][kR]);
   // the coarse set-up is queued while the library stream computes r
   if (!setup_aux()) {
     hipStreamSynchronize(aux);

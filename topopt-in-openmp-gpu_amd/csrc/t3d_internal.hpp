@@ -228,9 +228,10 @@ template <typename T>
 void launch_coarse_apply(const DevGrid &g, int l, const CoarseFactor &cf, const T *in, T *out);
 
 // BLAS-1 (FP64 accumulation in the reductions)
+// *result = a.b; also into the pinned host word `host` when given
 template <typename T>
-void launch_dot(const T *a, const T *b, int64_t n, double *partials,
-                double *result);
+void launch_dot(const T *a, const T *b, int64_t n, double *partials, double *result,
+                double *host = nullptr);
 void launch_axpy_cg(int64_t n, double *scal, double *u, double *r, const double *p,
                     const double *q);
 template <typename TO, typename TI>
