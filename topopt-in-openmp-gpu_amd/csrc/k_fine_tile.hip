@@ -610,8 +610,8 @@ __global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_tile_sweep(
   tile_sweep_body<T, LVL, MODE, BCSYM>(a, B, E, in, out, b, invD, omega, fix, dotp);
 }
 
-// fine level in FP32: workgroups of at most 512 threads, which lets the
-// compiler keep the kernel under 72 VGPRs (7 waves per SIMD)
+// fine level in FP32: workgroups of up to 1024 threads; 74 VGPRs, so two
+// 12-wave workgroups of 65 x 11 columns share a CU
 template <int MODE, bool BCSYM>
 __global__ __launch_bounds__(1024) void k_tile_sweep_f32(
     TileArgs a, Blocks<float> B, const float *__restrict__ E, const float *__restrict__ in,
