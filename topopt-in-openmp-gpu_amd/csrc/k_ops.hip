@@ -692,9 +692,12 @@ static double *kediag_for(int l) {
   element_matrices(0.3, l, KE.data());
   for (int c = 0; c < n * n * n; c++)
     for (int r = 0; r < 24; r++) dg[24 * c + r] = KE[576 * (size_t)c + 25 * r];
+  // dalloc clears the block on the library stream, so the copy goes on that
+  // stream too (after the clear) and completes before dg goes away
   double *d = (double *)dalloc(sizeof(double) * dg.size());
-  if (d && T3D_HIP(hipMemcpy(d, dg.data(), sizeof(double) * dg.size(),
-                             hipMemcpyHostToDevice)))
+  if (d && T3D_HIP(hipMemcpyAsync(d, dg.data(), sizeof(double) * dg.size(),
+                                  hipMemcpyHostToDevice, stream())) &&
+      T3D_HIP(hipStreamSynchronize(stream())))
     cache[l] = d;
   return cache[l];
 }
