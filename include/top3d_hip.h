@@ -245,7 +245,8 @@ void top3d_clear_error(void);
  * TOP3D_FP64  every vector, operator and reduction in double (plain form);
  * TOP3D_MIXED V-cycle (level operators, smoothers, transfers, level vectors,
  *             densities) in FP32; outer CG, its matvec, dot products and the
- *             coarse Cholesky in FP64.                                       */
+ *             coarse Cholesky in FP64 (r.z, whose z is the V-cycle's FP32
+ *             output, reads the FP32 copy of r and sums in FP64).            */
 #define TOP3D_FP64 0
 #define TOP3D_MIXED 1
 void top3d_set_precision(struct SolverDataBuffer *data, int mode);

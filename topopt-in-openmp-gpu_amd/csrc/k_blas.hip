@@ -134,16 +134,17 @@ __device__ __forceinline__ double sum_partials_all(const double *__restrict__ pa
   return block_sum_all(s);
 }
 
-// rho partials: r . z, z in the V-cycle's precision
-template <typename TZ>
-__global__ __launch_bounds__(kRedBlock) void k_dot_rz(const double *__restrict__ r,
+// rho partials: r . z, z in the V-cycle's precision (and r too in the mixed
+// form, whose FP32 z limits rho to FP32 accuracy anyway), summed in FP64
+template <typename TR, typename TZ>
+__global__ __launch_bounds__(kRedBlock) void k_dot_rz(const TR *__restrict__ r,
                                                       const TZ *__restrict__ z, int64_t n,
                                                       double *__restrict__ partials) {
   double s = 0;
 #pragma unroll 4
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
-    s += r[i] * (double)z[i];
+    s += (double)r[i] * (double)z[i];
   s = block_sum(s);
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
@@ -215,8 +216,8 @@ __global__ __launch_bounds__(kRedBlock) void k_axpy_fused(
 }
 
 template <typename TZ>
-void launch_dot_rz(const double *r, const TZ *z, int64_t n, double *partials) {
-  hipLaunchKernelGGL((k_dot_rz<TZ>), dim3(kRedGrid), dim3(kRedBlock), 0, stream(), r, z, n,
+void launch_dot_rz(const TZ *r, const TZ *z, int64_t n, double *partials) {
+  hipLaunchKernelGGL((k_dot_rz<TZ, TZ>), dim3(kRedGrid), dim3(kRedBlock), 0, stream(), r, z, n,
                      partials);
 }
 template <typename TZ>
@@ -239,7 +240,7 @@ void launch_axpy_fused(int64_t n, const double *pq_part, int npart, double *scal
                      host_rr);
 }
 template void launch_dot_rz<double>(const double *, const double *, int64_t, double *);
-template void launch_dot_rz<float>(const double *, const float *, int64_t, double *);
+template void launch_dot_rz<float>(const float *, const float *, int64_t, double *);
 template void launch_pupdate_fused<double>(int64_t, const double *, double *, double *,
                                            const double *, int, double *);
 template void launch_pupdate_fused<float>(int64_t, const double *, double *, double *,

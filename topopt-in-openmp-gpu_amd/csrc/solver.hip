@@ -345,7 +345,7 @@ int SolverState::solve(const double *x, const double *b, double *u, double tol, 
     pending = false;
     // rho = r.z; p = z + (rho / rho_old) p
     if (mixed) {
-      launch_dot_rz<float>(r, v32[0][kZ], n0, part_rz);
+      launch_dot_rz<float>(v32[0][kR], v32[0][kZ], n0, part_rz);
       launch_pupdate_fused<float>(n0, part_rz, scal, p, v32[0][kZ], iter == 0, u);
     } else {
       launch_dot_rz<double>(r, z, n0, part_rz);

@@ -246,8 +246,9 @@ void launch_pupdate(int64_t n, const double *scal, double *p, const double *z,
                     int first);
 // fused CG steps (partial sums consumed by the next kernel, kRedGrid = 1024
 // partials for launch_dot_rz and the r.r partials)
+// (r and z in the same precision: the FP32 copy of r in the mixed form)
 template <typename TZ>
-void launch_dot_rz(const double *r, const TZ *z, int64_t n, double *partials);
+void launch_dot_rz(const TZ *r, const TZ *z, int64_t n, double *partials);
 // p update; from the second iteration on also the previous u += alpha p
 template <typename TZ>
 void launch_pupdate_fused(int64_t n, const double *partials, double *scal, double *p,
