@@ -1,7 +1,7 @@
 """GPU parity of the MGCG state solve and the design loop against the CPU
 oracle, through the C ABI.
 
-Tolerances (the north-star bars): FP64 form -- same CG iteration count and
+Tolerances: FP64 form -- same CG iteration count and
 per-iteration compliance within 1e-8 relative of the FP64 oracle; mixed form
 (FP32 V-cycle, FP64 outer CG) -- compliance within 1e-5 relative."""
 import json
