@@ -268,6 +268,15 @@ void top3d_cantilever_load(const struct gridContext *gc, CTYPE *F);
 /* the line load of the MATLAB specification (top3dmgcg_reference.m:59):
  * F_z -= 1 at every node of the edge x = nelx, z = 0, added into F */
 void top3d_line_load(const struct gridContext *gc, CTYPE *F);
+/* the same edge with the specification's other profiles, F_z -= w_j at its
+ * node j = 0..nely (top3dmgcg_reference.m:57-59):
+ * TOP3D_LOAD_CONSTANT  w_j = 1 (as top3d_line_load);
+ * TOP3D_LOAD_END_HALF  w_j = 1, and 1/2 at the two end nodes;
+ * TOP3D_LOAD_SINE      w_j = sin(pi j / nely).                           */
+#define TOP3D_LOAD_CONSTANT 0
+#define TOP3D_LOAD_END_HALF 1
+#define TOP3D_LOAD_SINE 2
+void top3d_line_load_profile(const struct gridContext *gc, CTYPE *F, int profile);
 /* supports that setFixedDof_halo (stencil_grid_utility.c:7-65), and so
  * setupGC, builds from this call on:
  * TOP3D_SUPPORT_STRIPS  two clamped strips of the x=0 face, y in

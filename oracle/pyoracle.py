@@ -53,6 +53,7 @@ def load(typing="fp64"):
         "or_set_bcsym": (None, [P, c_int]),
         "or_set_support": (None, [P, c_int]),
         "or_line_load": (None, [P, P]),
+        "or_line_load_profile": (None, [P, P, c_int]),
         "or_padding": (None, [P, c_int, P, P, P, P, P]),
         "or_nfixed": (c_int, [P, c_int]),
         "or_fixed": (None, [P, c_int, P]),
@@ -155,9 +156,10 @@ class Grid:
         self.lib.or_load(self.h, _p(F))
         return F
 
-    def line_load(self):
+    def line_load(self, profile=0):
+        """0: constant, 1: half weight at the two end nodes, 2: sine."""
         F = np.zeros(self.ndof(0), self.ct)
-        self.lib.or_line_load(self.h, _p(F))
+        self.lib.or_line_load_profile(self.h, _p(F), profile)
         return F
 
     # ---- operators --------------------------------------------------------

@@ -261,18 +261,26 @@ void or_destroy(or_grid *g) {
   free(g);
 }
 
-/* Line load of the MATLAB specification (top3dmgcg_reference.m:59, "constant
- * load"): F_z = -1 at every node of the edge x = nelx, z = 0, i.e. the halo
- * nodes (nelx+1, j, 1) for j = 1..nely+1.                                   */
-void or_line_load(const or_grid *g, or_ctype *F) {
+/* Line loads of the MATLAB specification (top3dmgcg_reference.m:57-59) on
+ * the edge x = nelx, z = 0, i.e. the halo nodes (nelx+1, jj+1, 1) for
+ * jj = 0..nely: F_z = -w_jj with w = 1 (profile 0, "constant load"), w = 1
+ * but 1/2 at the two ends (profile 1), or w = sin(pi jj / nely) (profile 2,
+ * "sine load").                                                             */
+void or_line_load_profile(const or_grid *g, or_ctype *F, int profile) {
   int ncell, wx, wy, wz;
   int64_t ndof;
   or_padding(g, 0, &ncell, &wx, &wy, &wz, &ndof);
   memset(F, 0, sizeof(or_ctype) * ndof);
-  const int64_t i = g->nelx + 1, k = 1;
-  for (int64_t j = 1; j <= g->nely + 1; j++)
-    F[3 * (i * wy * wz + k * wy + j) + 2] = -1;
+  const int64_t i = g->nelx + 1, k = 1, ny = g->nely;
+  for (int64_t jj = 0; jj <= ny; jj++) {
+    double w = 1.0;
+    if (profile == 1 && (jj == 0 || jj == ny)) w = 0.5;
+    if (profile == 2) w = sin(M_PI * (double)jj / (double)ny);
+    F[3 * (i * wy * wz + k * wy + jj + 1) + 2] = (or_ctype)(-w);
+  }
 }
+
+void or_line_load(const or_grid *g, or_ctype *F) { or_line_load_profile(g, F, 0); }
 
 /* Cantilever load, stencil_optimization.c:39-95: F_z = -1/(4*count) on the 4
  * z=0-face nodes of each element (i,j) inside the disc around

@@ -90,6 +90,7 @@ void or_ke(double nu, int l, int npts, double *KE);
 void or_ke_copy(const or_grid *g, int l, double *out);
 void or_load(const or_grid *g, or_ctype *F);
 void or_line_load(const or_grid *g, or_ctype *F);
+void or_line_load_profile(const or_grid *g, or_ctype *F, int profile);
 
 void or_matvec0(const or_grid *g, const or_dtype *x, const or_ctype *in,
                 or_ctype *out);

@@ -41,3 +41,24 @@ def test_face_clamp_line_load_solve(precision, bc):
         assert abs(c - co) <= 1e-5 * abs(co)
     g.close()
     o.close()
+
+
+@pytest.mark.parametrize("profile", [1, 2])
+def test_line_load_profiles_solve(profile):
+    # end-weighted and sine edge loads (top3dmgcg_reference.m:57-58), FP64
+    nelx, nely, nelz, nl = 32, 16, 16, 4
+    print(f"[gpu supports] line load profile {profile}", flush=True)
+    g = top3d_amd.Grid(nelx, nely, nelz, nl, support="face")
+    o = pyoracle.Grid(nelx, nely, nelz, nl, support="face")
+    F = g.line_load(profile)
+    Fo = o.line_load(profile)
+    np.testing.assert_allclose(F.cpu().numpy(), Fo, rtol=1e-15, atol=1e-17)
+    x = g.density(fill=float(np.float32(0.2)))
+    U = g.zeros(0)
+    it, res = g.solve(x, F, U)
+    Uo = np.zeros(o.ndof(0))
+    ito, reso = o.solve(x.cpu().numpy(), Fo, Uo)
+    assert it == ito
+    assert np.linalg.norm(U.cpu().numpy() - Uo) <= 1e-9 * np.linalg.norm(Uo)
+    g.close()
+    o.close()

@@ -91,3 +91,34 @@ def test_library_line_load_matches_oracle():
     lib.top3d_line_load(ctypes.byref(gc), F.ctypes.data_as(ctypes.c_void_p))
     assert np.array_equal(F, o.line_load())
     o.close()
+
+
+@pytest.mark.parametrize("profile", [1, 2])
+def test_oracle_line_load_profiles(profile):
+    # end-weighted: total nely; sine: sum_j sin(pi j / n) = cot(pi / (2 n)),
+    # zero at both ends and mirror-symmetric about the middle node
+    o = pyoracle.Grid(*SHAPE)
+    F, F0 = o.line_load(profile), o.line_load(0)
+    n = SHAPE[1]
+    nodes = np.flatnonzero(F0)  # the edge's z components, j = 0..nely in order
+    w = -F[nodes]
+    assert np.all(F[np.setdiff1d(np.arange(F.size), nodes)] == 0)
+    if profile == 1:
+        assert w.sum() == n and w[0] == w[-1] == 0.5 and np.all(w[1:-1] == 1)
+    else:
+        assert abs(w.sum() - 1 / np.tan(np.pi / (2 * n))) <= 1e-13 * n
+        assert abs(w[0]) <= 1e-15 and abs(w[-1]) <= 1e-15
+        np.testing.assert_allclose(w, w[::-1], rtol=0, atol=1e-15)
+        assert w[n // 2] == pytest.approx(1.0, abs=1e-15)
+    o.close()
+
+
+@pytest.mark.parametrize("profile", [0, 1, 2])
+def test_library_line_load_profiles_match_oracle(profile):
+    lib = top3d_amd.library()
+    gc, _ = host_context(*SHAPE)
+    o = pyoracle.Grid(*SHAPE)
+    F = np.zeros(o.ndof(0))
+    lib.top3d_line_load_profile(ctypes.byref(gc), F.ctypes.data_as(ctypes.c_void_p), profile)
+    np.testing.assert_allclose(F, o.line_load(profile), rtol=1e-15, atol=1e-17)
+    o.close()

@@ -48,9 +48,18 @@ void top3d_cantilever_load(const struct gridContext *gc, CTYPE *F) {
 }
 
 void top3d_line_load(const struct gridContext *gc, CTYPE *F) {
+  top3d_line_load_profile(gc, F, TOP3D_LOAD_CONSTANT);
+}
+
+void top3d_line_load_profile(const struct gridContext *gc, CTYPE *F, int profile) {
   const uint64_t wy = gc->wrapy, wz = gc->wrapz;
-  const uint64_t i = gc->nelx + 1, k = 1;
-  for (uint64_t j = 1; j < gc->nely + 2; j++) F[3 * (i * wy * wz + k * wy + j) + 2] -= 1.0;
+  const uint64_t i = gc->nelx + 1, k = 1, ny = gc->nely;
+  for (uint64_t jj = 0; jj <= ny; jj++) {
+    double w = 1.0;
+    if (profile == TOP3D_LOAD_END_HALF && (jj == 0 || jj == ny)) w = 0.5;
+    if (profile == TOP3D_LOAD_SINE) w = std::sin(M_PI * (double)jj / (double)ny);
+    F[3 * (i * wy * wz + k * wy + (jj + 1)) + 2] -= w;
+  }
 }
 
 void top3dmgcg(const uint_fast32_t nelx, const uint_fast32_t nely, const uint_fast32_t nelz,

@@ -136,6 +136,7 @@ SIGNATURES = {
     "top3d_set_default_bc_mode": (None, [c_int]),
     "top3d_cantilever_load": (None, [P, P]),
     "top3d_line_load": (None, [P, P]),
+    "top3d_line_load_profile": (None, [P, P, c_int]),
     "top3d_set_default_support": (None, [c_int]),
     "top3d_get_default_support": (c_int, []),
     "top3d_map": (c_int, [P, c_u64]),
@@ -299,9 +300,12 @@ class Grid:
         self.lib.top3d_cantilever_load(ctypes.byref(self.gc), F.ctypes.data_as(c_void_p))
         return self.torch.from_numpy(F).to(self.device)
 
-    def line_load(self):
+    def line_load(self, profile=0):
+        """Edge load of the MATLAB specification; profile 0 constant, 1 half
+        weight at the two end nodes, 2 sine."""
         F = np.zeros(self.ndof(0))
-        self.lib.top3d_line_load(ctypes.byref(self.gc), F.ctypes.data_as(c_void_p))
+        self.lib.top3d_line_load_profile(ctypes.byref(self.gc), F.ctypes.data_as(c_void_p),
+                                         profile)
         return self.torch.from_numpy(F).to(self.device)
 
     def ke(self, l):
