@@ -651,7 +651,7 @@ void update_solution(const DTYPE *x, DTYPE *xnew, const DTYPE *dv, const DTYPE *
       !vv.open(dv, sizeof(double) * nelem, true, false) ||
       !vc.open(dc, sizeof(double) * nelem, true, false))
     return;
-  double *scratch = (double *)dalloc(sizeof(double) * (1024 + 16));
+  double *scratch = (double *)dalloc(sizeof(double) * kOcScratchDoubles);
   oc_update((const double *)vx.dev, (double *)vn.dev, (const double *)vv.dev,
             (const double *)vc.dev, nelem, gval, scratch);
   dfree(scratch);

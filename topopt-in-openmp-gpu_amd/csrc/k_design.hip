@@ -201,66 +201,149 @@ void launch_change(int64_t n, double *x, const double *xnew, double *partials, d
 // Bisection on lambda kept on the device: each step evaluates xnew(lmid) and
 // sum dv (xnew - x) over the padded array, then one thread updates l1/l2.
 // state[0] l1, [1] l2, [2] lmid, [3] done flag, [4] g.
-__global__ __launch_bounds__(kBS) void k_oc_step(int64_t n, const double *__restrict__ x,
-                                                 double *__restrict__ xnew,
+// The bisection of gpu_definitions.c:127-157 evaluates one multiplier per
+// pass over the elements.  Here one pass evaluates the kOcDepth levels of the
+// bisection tree below the current interval (2^kOcDepth - 1 multipliers), and
+// a one-block kernel walks the tree with the sums, so the multipliers visited
+// are those of the step-by-step loop.  The element arrays are read once per
+// kOcDepth steps.  The candidate x_e sqrt(-dc_e / (dv_e lmid)) is formed as
+// x_e sqrt(-dc_e / dv_e) * (1 / sqrt(lmid)), one division and square root per
+// element instead of per multiplier (a few ulp from the one-step form).
+constexpr int kOcDepth = 3;
+constexpr int kOcCand = (1 << kOcDepth) - 1;
+constexpr int kOcChunk = 24;  // passes between host checks (72 bisection steps)
+static_assert(kOcScratchDoubles >= 8 + kOcCand * kPartials, "OC scratch size");
+
+// Multipliers of the tree below [l1, l2], in heap order: node n has the
+// lower half as child 2n+1 and the upper half as child 2n+2.
+__device__ __forceinline__ void oc_tree(double l1, double l2, double (&mid)[kOcCand]) {
+  double lo[kOcCand], hi[kOcCand];
+  lo[0] = l1;
+  hi[0] = l2;
+#pragma unroll
+  for (int n = 0; n < kOcCand; n++) {
+    mid[n] = 0.5 * (hi[n] + lo[n]);
+    if (2 * n + 2 < kOcCand) {
+      lo[2 * n + 1] = lo[n];
+      hi[2 * n + 1] = mid[n];
+      lo[2 * n + 2] = mid[n];
+      hi[2 * n + 2] = hi[n];
+    }
+  }
+}
+
+// xnew_e for a multiplier with rl = 1 / sqrt(lmid), a = x_e sqrt(-dc_e / dv_e):
+// move limit 0.2, bounds [0, 1].
+__device__ __forceinline__ double oc_x(double xi, double a, double rl) {
+  const double move = 0.2;
+  return fmax(0.0, fmax(xi - move, fmin(1.0, fmin(xi + move, a * rl))));
+}
+
+// state: [0] l1, [1] l2, [3] converged flag, [4] gval, [5] last multiplier,
+// [6] steps taken.  partials: kOcCand rows of gridDim.x block sums.
+__global__ __launch_bounds__(kBS) void k_oc_pass(int64_t n, const double *__restrict__ x,
                                                  const double *__restrict__ dv,
                                                  const double *__restrict__ dc,
                                                  const double *__restrict__ state,
                                                  double *__restrict__ partials) {
   if (state[3] != 0.0) return;
-  const double lmid = 0.5 * (state[1] + state[0]);
-  const double move = 0.2;
+  double rl[kOcCand];
+  oc_tree(state[0], state[1], rl);
+  double s[kOcCand];
+#pragma unroll
+  for (int c = 0; c < kOcCand; c++) {
+    rl[c] = 1.0 / sqrt(rl[c]);
+    s[c] = 0.0;
+  }
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double xi = x[i], dvi = dv[i];
+    const double a = xi * sqrt(-dc[i] / dvi);
+#pragma unroll
+    for (int c = 0; c < kOcCand; c++) s[c] += dvi * (oc_x(xi, a, rl[c]) - xi);
+  }
+#pragma unroll
+  for (int c = 0; c < kOcCand; c++) {
+    const double t = blk_sum(s[c]);
+    if (threadIdx.x == 0) partials[(int64_t)c * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// One workgroup of kOcCand wavefronts, wave c summing multiplier c's
+// partials; then up to kOcDepth bisection steps down the tree, stopping where
+// the interval has converged.
+__global__ __launch_bounds__(64 * kOcCand) void k_oc_walk(const double *__restrict__ partials,
+                                                          int np, double *__restrict__ state) {
+  if (state[3] != 0.0) return;
+  __shared__ double S[kOcCand];
+  const int c = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double s = 0;
+  for (int i = lane; i < np; i += 64) s += partials[c * np + i];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) S[c] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double l1 = state[0], l2 = state[1];
+    int node = 0;
+    for (int d = 0; d < kOcDepth; d++) {
+      const double lmid = 0.5 * (l2 + l1);
+      state[5] = lmid;
+      state[6] += 1.0;
+      if (S[node] + state[4] > 0) {
+        l1 = lmid;
+        node = 2 * node + 2;
+      } else {
+        l2 = lmid;
+        node = 2 * node + 1;
+      }
+      if (!((l2 - l1) / (l1 + l2) > 1e-6)) {
+        state[3] = 1.0;
+        break;
+      }
+    }
+    state[0] = l1;
+    state[1] = l2;
+  }
+}
+
+// xnew for the last multiplier the walk visited.
+__global__ __launch_bounds__(kBS) void k_oc_write(int64_t n, const double *__restrict__ x,
+                                                  double *__restrict__ xnew,
+                                                  const double *__restrict__ dv,
+                                                  const double *__restrict__ dc,
+                                                  const double *__restrict__ state) {
+  const double rl = 1.0 / sqrt(state[5]);
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const double xi = x[i];
-    const double cand = xi * sqrt(-dc[i] / (dv[i] * lmid));
-    const double v = fmax(0.0, fmax(xi - move, fmin(1.0, fmin(xi + move, cand))));
-    xnew[i] = v;
-    s += dv[i] * (v - xi);
-  }
-  s = blk_sum(s);
-  if (threadIdx.x == 0) partials[blockIdx.x] = s;
-}
-
-__global__ void k_oc_bisect(const double *__restrict__ partials, int np,
-                            double *__restrict__ state) {
-  if (state[3] != 0.0) return;
-  double s = 0;
-  for (int i = threadIdx.x; i < np; i += blockDim.x) s += partials[i];
-  s = blk_sum(s);
-  if (threadIdx.x == 0) {
-    const double l1 = state[0], l2 = state[1];
-    const double lmid = 0.5 * (l2 + l1);
-    const double gt = s + state[4];
-    double n1 = l1, n2 = l2;
-    if (gt > 0) n1 = lmid; else n2 = lmid;
-    state[0] = n1;
-    state[1] = n2;
-    if (!((n2 - n1) / (n1 + n2) > 1e-6)) state[3] = 1.0;
+    xnew[i] = oc_x(xi, xi * sqrt(-dc[i] / dv[i]), rl);
   }
 }
 
-// Runs the bisection; the loop condition is checked on the host every 16
-// steps (steps after convergence are skipped on the device).  Returns the
-// number of bisection steps or -1.
+// Runs the bisection, checking on the host every kOcChunk passes (passes
+// after convergence return at once).  xnew is written after each chunk for
+// the last multiplier, so a converged chunk needs no further launch.
+// Returns the number of bisection steps or -1.
 int oc_update(const double *x, double *xnew, const double *dv, const double *dc,
               int64_t nelem, double gval, double *scratch) {
   double *state = scratch;          // 8 doubles
-  double *partials = scratch + 8;   // kPartials doubles
+  double *partials = scratch + 8;   // kOcCand * kPartials doubles
   const double init[8] = {0.0, 1e9, 0.0, 0.0, gval, 0, 0, 0};
   hipMemcpyAsync(state, init, sizeof(init), hipMemcpyHostToDevice, stream());
   double h[8];
-  for (int it = 0; it < 4096; it += 16) {
-    for (int q = 0; q < 16; q++) {
-      hipLaunchKernelGGL(k_oc_step, dim3(kPartials), dim3(kBS), 0, stream(), nelem, x, xnew,
-                         dv, dc, state, partials);
-      hipLaunchKernelGGL(k_oc_bisect, dim3(1), dim3(kBS), 0, stream(), partials, kPartials,
-                         state);
+  for (int it = 0; it < 4096; it += kOcChunk * kOcDepth) {
+    for (int q = 0; q < kOcChunk; q++) {
+      hipLaunchKernelGGL(k_oc_pass, dim3(kPartials), dim3(kBS), 0, stream(), nelem, x, dv, dc,
+                         state, partials);
+      hipLaunchKernelGGL(k_oc_walk, dim3(1), dim3(64 * kOcCand), 0, stream(), partials,
+                         kPartials, state);
     }
+    hipLaunchKernelGGL(k_oc_write, dim3(kPartials), dim3(kBS), 0, stream(), nelem, x, xnew, dv,
+                       dc, state);
     hipMemcpyAsync(h, state, sizeof(h), hipMemcpyDeviceToHost, stream());
     if (!T3D_HIP(hipStreamSynchronize(stream()))) return -1;
-    if (h[3] != 0.0) return it + 16;
+    if (h[3] != 0.0) return (int)h[6];
   }
   return -1;
 }

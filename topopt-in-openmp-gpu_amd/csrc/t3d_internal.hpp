@@ -276,6 +276,8 @@ void launch_volume(const DevGrid &g, const double *x, double *partials,
                    double *result);
 void launch_change(int64_t n, double *x, const double *xnew, double *partials,
                    double *result);
+// scratch: kOcScratchDoubles doubles of device memory.
+constexpr int kOcScratchDoubles = 8 + 15 * 1024;
 int oc_update(const double *x, double *xnew, const double *dv,
               const double *dc, int64_t nelem, double gval, double *scratch);
 
