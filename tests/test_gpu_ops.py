@@ -189,3 +189,16 @@ def test_sensitivity_filter_volume_change_oc(pair):
     ch = g.change(xc, xnew)
     assert ch == pytest.approx(float(np.float32(np.abs(xo - ref_new).max())), rel=1e-6)
     assert torch.equal(xc, xnew)
+
+
+@pytest.mark.parametrize("rmin", [1.2, 2.0, 2.5, 4.5])
+def test_filter_radii(pair, rmin):
+    """Filter and its transpose at radii with 7, 27 and 81 non-zero window
+    weights (tap-list kernels of 8, 28 and 240 unrolled taps) and with more
+    than 240 (the full-window kernel)."""
+    g, o, xg, xo, rng = pair
+    inter = g.interior_elements()
+    assert relerr(g.filter(rmin, xg).cpu().numpy()[inter], o.filter(rmin, xo)[inter]) <= 1e-14
+    v = g.density(rng.uniform(-1, 1, inter.size))
+    assert relerr(g.filter_grad(rmin, v).cpu().numpy()[inter],
+                  o.filter_grad(rmin, v.cpu().numpy())[inter]) <= 1e-14

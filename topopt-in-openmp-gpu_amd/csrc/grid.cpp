@@ -284,6 +284,7 @@ static DevGrid *build_dev_grid(struct gridContext *gc, int nl) {
                            hipMemcpyHostToDevice, stream()));
     T3D_HIP(hipStreamSynchronize(stream()));
   }
+  g->wh0 = hadamard_blocks(gc->precomputedKE[0], g->M0);
   for (int l = 0; l < nl; l++) upload_fixed(gc, g, l);
   std::lock_guard<std::mutex> lk(g_gmu);
   g_grids[(const void *)gc->precomputedKE] = g;

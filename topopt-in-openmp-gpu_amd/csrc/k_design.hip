@@ -1,7 +1,9 @@
 // Design-loop kernels: compliance and sensitivity, the density filter and
 // its transpose, volume, change and the optimality-criteria update.
 #include <algorithm>
+#include <climits>
 #include <cmath>
+#include <cstring>
 
 #include "t3d_internal.hpp"
 
@@ -86,12 +88,78 @@ __global__ __launch_bounds__(kBS) void k_sensitivity(
   if (threadIdx.x == 0) partials[blockIdx.x] = csum;
 }
 
+// The same quantities with u_e^T KE u_e = h^T (T KE T^T / 64) h, h = T u_e the
+// per-component 8-point Walsh-Hadamard transform of the element's nodal
+// values: 72 additions and 8 3x3 blocks instead of 576 multiply-adds with KE
+// read per element.  Element coordinates in 32-bit arithmetic (ne < 2^31).
+struct WhBlocks {
+  double m[8][3][3];
+};
+
+__global__ __launch_bounds__(kBS) void k_sensitivity_wh(
+    int nelx, int nely, int nelz, int wy, int wz, int few, int fez, double E0, double Emin,
+    const WhBlocks B, const double *__restrict__ x, const double *__restrict__ u,
+    double *__restrict__ dc, double *__restrict__ partials) {
+  const uint32_t ne = (uint32_t)nelx * nely * nelz;
+  double csum = 0.0;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < ne; t += gridDim.x * blockDim.x) {
+    const uint32_t ty = t / (uint32_t)nely, tz = ty / (uint32_t)nelz;
+    const int j = (int)(t - ty * nely) + 1, k = (int)(ty - tz * nelz) + 1, i = (int)tz + 1;
+    double h[8][3];  // [p][c], p = px | py << 1 | pz << 2
+#pragma unroll
+    for (int p = 0; p < 8; p++) {
+      const int64_t nd =
+          3 * (((int64_t)(i + (p & 1)) * wz + (k + (p >> 2))) * wy + (j + ((p >> 1) & 1)));
+      h[p][0] = u[nd];
+      h[p][1] = u[nd + 1];
+      h[p][2] = u[nd + 2];
+    }
+#pragma unroll
+    for (int bit = 1; bit < 8; bit <<= 1)
+#pragma unroll
+      for (int p = 0; p < 8; p++)
+        if (!(p & bit))
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            const double a = h[p][c], b = h[p | bit][c];
+            h[p][c] = a + b;
+            h[p | bit][c] = a - b;
+          }
+    double ce = 0.0;
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+      const double hs[3] = {h[s ^ 1][0], h[s ^ 2][1], h[s ^ 4][2]};
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        double tr = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; d++) tr += B.m[s][c][d] * hs[d];
+        ce += hs[c] * tr;
+      }
+    }
+    const int64_t e = ((int64_t)i * fez + k) * few + j;
+    const double xe = x[e];
+    csum += ce * (Emin + xe * xe * xe * (E0 - Emin));
+    dc[e] = ce * (-3.0 * (E0 - Emin) * xe * xe);
+  }
+  csum = blk_sum(csum);
+  if (threadIdx.x == 0) partials[blockIdx.x] = csum;
+}
+
 void launch_sensitivity(const DevGrid &g, const double *x, const double *u, double *dc,
                         double *partials, double *c) {
   const Level &L = g.lv[0];
-  hipLaunchKernelGGL(k_sensitivity, dim3(kPartials), dim3(kBS), 0, stream(), L.nelx, L.nely,
-                     L.nelz, L.wy, L.wz, g.ewy, g.ewz, g.E0, g.Emin, g.KE[0], x, u, dc,
-                     partials);
+  if (g.wh0 && (int64_t)L.nelx * L.nely * L.nelz < (int64_t)INT32_MAX) {
+    WhBlocks B;
+    std::memcpy(B.m, g.M0, sizeof(B.m));
+    hipLaunchKernelGGL(k_sensitivity_wh, dim3(kPartials), dim3(kBS), 0, stream(), L.nelx,
+                       L.nely, L.nelz, L.wy, L.wz, g.ewy, g.ewz, g.E0, g.Emin, B, x, u, dc,
+                       partials);
+  } else {
+    hipLaunchKernelGGL(k_sensitivity, dim3(kPartials), dim3(kBS), 0, stream(), L.nelx, L.nely,
+                       L.nelz, L.wy, L.wz, g.ewy, g.ewz, g.E0, g.Emin, g.KE[0], x, u, dc,
+                       partials);
+  }
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBS), 0, stream(), partials, kPartials, 0,
                      c);
 }
@@ -133,24 +201,97 @@ __global__ __launch_bounds__(kBS) void k_filter(int nelx, int nely, int nelz, in
   if (MODE == 2) out[e1] = acc;
 }
 
-void launch_filter(const DevGrid &g, double rmin, const double *rho, double *out) {
+// The non-zero weights of the window, in the loop order of the kernel above
+// (i2, then k2, then j2 ascending).  The window's zero weights add nothing to
+// either sum, so leaving them out keeps both sums bit for bit; the weights use
+// the same expression as the loop.  rmin 1.5: 19 of 64 terms.  The list is
+// padded to the kernel's unrolled length with zero weights at offset 0, which
+// add +0 to both sums.
+constexpr int kFilterMaxTaps = 240;
+struct FilterTaps {
+  int8_t di[kFilterMaxTaps], dk[kFilterMaxTaps], dj[kFilterMaxTaps];
+  double w[kFilterMaxTaps];
+};
+
+static int filter_taps(double rmin, FilterTaps &t) {
+  const int cr = (int)std::ceil(rmin);
+  int n = 0;
+  for (int di = -(cr - 1); di <= cr; di++)
+    for (int dk = -(cr - 1); dk <= cr; dk++)
+      for (int dj = -(cr - 1); dj <= cr; dj++) {
+        const double w = std::fmax(0.0, rmin - std::sqrt((double)(di * di + dj * dj + dk * dk)));
+        if (w == 0.0) continue;
+        if (n == kFilterMaxTaps) return -1;
+        t.di[n] = (int8_t)di;
+        t.dk[n] = (int8_t)dk;
+        t.dj[n] = (int8_t)dj;
+        t.w[n++] = w;
+      }
+  for (int q = n; q < kFilterMaxTaps; q++) {
+    t.di[q] = t.dk[q] = t.dj[q] = 0;
+    t.w[q] = 0.0;
+  }
+  return n;
+}
+
+template <int MODE, int NT>
+__global__ __launch_bounds__(kBS) void k_filter_taps(int nelx, int nely, int nelz, int few,
+                                                     int fez, const FilterTaps taps,
+                                                     const double *__restrict__ in,
+                                                     double *__restrict__ out) {
+  const uint32_t ne = (uint32_t)nelx * nely * nelz;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ne) return;
+  const uint32_t ty = t / (uint32_t)nely, tz = ty / (uint32_t)nelz;
+  const int j1 = (int)(t - ty * nely) + 1, k1 = (int)(ty - tz * nelz) + 1, i1 = (int)tz + 1;
+  const int64_t e1 = ((int64_t)i1 * fez + k1) * few + j1;
+  double acc = 0.0, ws = 0.0;
+  // branch-free: a tap outside the grid reads the element itself with weight
+  // +0, so every load of the window can be in flight at once
+#pragma unroll
+  for (int q = 0; q < NT; q++) {
+    const int i2 = i1 + taps.di[q], k2 = k1 + taps.dk[q], j2 = j1 + taps.dj[q];
+    const bool in_grid = i2 >= 1 && i2 <= nelx && k2 >= 1 && k2 <= nelz && j2 >= 1 && j2 <= nely;
+    const double w = in_grid ? taps.w[q] : 0.0;
+    const int64_t off = in_grid ? ((int64_t)taps.di[q] * fez + taps.dk[q]) * few + taps.dj[q] : 0;
+    if (MODE != 1) acc += w * in[e1 + off];
+    ws += w;
+  }
+  if (MODE == 0) out[e1] = acc / ws;
+  if (MODE == 1) out[e1] = in[e1] / ws;
+  if (MODE == 2) out[e1] = acc;
+}
+
+template <int MODE>
+static void filter_pass(const DevGrid &g, double rmin, const double *in, double *out) {
   const Level &L = g.lv[0];
   const int64_t ne = L.elements();
-  hipLaunchKernelGGL((k_filter<0>), dim3((ne + kBS - 1) / kBS), dim3(kBS), 0, stream(),
-                     L.nelx, L.nely, L.nelz, g.ewy, g.ewz, rmin, (int)std::ceil(rmin), rho,
-                     out);
+  const dim3 grid((ne + kBS - 1) / kBS);
+  FilterTaps taps;
+  const int n = ne < (int64_t)INT32_MAX ? filter_taps(rmin, taps) : -1;
+#define T3D_FILTER_TAPS(NT)                                                                  \
+  hipLaunchKernelGGL((k_filter_taps<MODE, NT>), grid, dim3(kBS), 0, stream(), L.nelx, L.nely, \
+                     L.nelz, g.ewy, g.ewz, taps, in, out)
+  if (n >= 0 && n <= 8) T3D_FILTER_TAPS(8);
+  else if (n >= 0 && n <= 20) T3D_FILTER_TAPS(20);
+  else if (n >= 0 && n <= 28) T3D_FILTER_TAPS(28);
+  else if (n >= 0 && n <= 64) T3D_FILTER_TAPS(64);
+  else if (n >= 0) T3D_FILTER_TAPS(kFilterMaxTaps);
+  else
+    hipLaunchKernelGGL((k_filter<MODE>), grid, dim3(kBS), 0, stream(), L.nelx, L.nely, L.nelz,
+                       g.ewy, g.ewz, rmin, (int)std::ceil(rmin), in, out);
+#undef T3D_FILTER_TAPS
+}
+
+void launch_filter(const DevGrid &g, double rmin, const double *rho, double *out) {
+  filter_pass<0>(g, rmin, rho, out);
 }
 
 void launch_filter_grad(const DevGrid &g, double rmin, const double *v, double *tmp,
                         double *out) {
-  const Level &L = g.lv[0];
-  const int64_t ne = L.elements();
   launch_fill<double>(g.nelem, tmp, 0.0);
-  hipLaunchKernelGGL((k_filter<1>), dim3((ne + kBS - 1) / kBS), dim3(kBS), 0, stream(),
-                     L.nelx, L.nely, L.nelz, g.ewy, g.ewz, rmin, (int)std::ceil(rmin), v, tmp);
-  hipLaunchKernelGGL((k_filter<2>), dim3((ne + kBS - 1) / kBS), dim3(kBS), 0, stream(),
-                     L.nelx, L.nely, L.nelz, g.ewy, g.ewz, rmin, (int)std::ceil(rmin), tmp,
-                     out);
+  filter_pass<1>(g, rmin, v, tmp);
+  filter_pass<2>(g, rmin, tmp, out);
 }
 
 // ---- volume (gpu_definitions.c:94-110) ------------------------------------
@@ -198,9 +339,6 @@ void launch_change(int64_t n, double *x, const double *xnew, double *partials, d
 }
 
 // ---- optimality criteria (gpu_definitions.c:127-157) ----------------------
-// Bisection on lambda kept on the device: each step evaluates xnew(lmid) and
-// sum dv (xnew - x) over the padded array, then one thread updates l1/l2.
-// state[0] l1, [1] l2, [2] lmid, [3] done flag, [4] g.
 // The bisection of gpu_definitions.c:127-157 evaluates one multiplier per
 // pass over the elements.  Here one pass evaluates the kOcDepth levels of the
 // bisection tree below the current interval (2^kOcDepth - 1 multipliers), and

@@ -71,6 +71,10 @@ struct DevGrid {
   double *KE[2] = {};                  // device KE_0 (576), KE_1 (8*576)
   float *KEf[2] = {};
   double *KE2 = nullptr;               // device KE_2 (64*576), nl >= 3
+  // 3x3 blocks of KE_0 in the per-component Walsh-Hadamard basis of the
+  // element's nodal values (valid if wh0)
+  bool wh0 = false;
+  double M0[8][3][3] = {};
   void *scratch[8] = {};               // lazily allocated work arrays of the API
   size_t scratch_bytes[8] = {};
   LevelArgs args(int l) const;
@@ -284,5 +288,8 @@ int oc_update(const double *x, double *xnew, const double *dv,
 // ---------------------------------------------------------------------------
 // Host-side KE generation (same quadrature results as local_matrix.c).
 void element_matrices(double nu, int l, double *KE);
+// M[s][c][d]: the 3x3 blocks of T KE T^T / 64 (T: per-component 8-point
+// Walsh-Hadamard transform); false if KE lacks the reflection structure.
+bool hadamard_blocks(const double *KE, double M[8][3][3]);
 
 }  // namespace t3d
