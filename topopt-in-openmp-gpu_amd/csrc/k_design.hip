@@ -262,11 +262,71 @@ __global__ __launch_bounds__(kBS) void k_filter_taps(int nelx, int nely, int nel
   if (MODE == 2) out[e1] = acc;
 }
 
+// Radius at most 2 (window offsets -1..2 per axis): the offsets with
+// di^2 + dk^2 + dj^2 <= D2MAX are compile-time constants, the weight depends
+// only on that squared distance (w2[d2], same expression as the loop), and
+// the per-axis range checks are formed once per element.
+template <int MODE, int D2MAX>
+__global__ __launch_bounds__(kBS) void k_filter_r2(int nelx, int nely, int nelz, int few,
+                                                   int fez, double w0, double w1, double w2,
+                                                   double w3, const double *__restrict__ in,
+                                                   double *__restrict__ out) {
+  const uint32_t ne = (uint32_t)nelx * nely * nelz;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ne) return;
+  const uint32_t ty = t / (uint32_t)nely, tz = ty / (uint32_t)nelz;
+  const int j1 = (int)(t - ty * nely) + 1, k1 = (int)(ty - tz * nelz) + 1, i1 = (int)tz + 1;
+  const int64_t e1 = ((int64_t)i1 * fez + k1) * few + j1;
+  const int64_t plane = (int64_t)fez * few;
+  const double wd[4] = {w0, w1, w2, w3};
+  bool oi[4], ok[4], oj[4];  // offset d = index - 1 inside the grid
+#pragma unroll
+  for (int d = 0; d < 4; d++) {
+    oi[d] = i1 + d - 1 >= 1 && i1 + d - 1 <= nelx;
+    ok[d] = k1 + d - 1 >= 1 && k1 + d - 1 <= nelz;
+    oj[d] = j1 + d - 1 >= 1 && j1 + d - 1 <= nely;
+  }
+  double acc = 0.0, ws = 0.0;
+#pragma unroll
+  for (int di = -1; di <= 2; di++)
+#pragma unroll
+    for (int dk = -1; dk <= 2; dk++)
+#pragma unroll
+      for (int dj = -1; dj <= 2; dj++) {
+        const int d2 = di * di + dk * dk + dj * dj;
+        if (d2 > D2MAX) continue;
+        const bool in_grid = oi[di + 1] && ok[dk + 1] && oj[dj + 1];
+        const double w = in_grid ? wd[d2] : 0.0;
+        if (MODE != 1) acc += w * in[in_grid ? e1 + di * plane + dk * few + dj : e1];
+        ws += w;
+      }
+  if (MODE == 0) out[e1] = acc / ws;
+  if (MODE == 1) out[e1] = in[e1] / ws;
+  if (MODE == 2) out[e1] = acc;
+}
+
 template <int MODE>
 static void filter_pass(const DevGrid &g, double rmin, const double *in, double *out) {
   const Level &L = g.lv[0];
   const int64_t ne = L.elements();
   const dim3 grid((ne + kBS - 1) / kBS);
+  if (ne < (int64_t)INT32_MAX && rmin > 0.0 && rmin <= 2.0) {
+    double w[4];
+    int d2max = -1;
+    for (int d2 = 0; d2 < 4; d2++) {
+      w[d2] = std::fmax(0.0, rmin - std::sqrt((double)d2));
+      if (w[d2] != 0.0) d2max = d2;
+    }
+#define T3D_FILTER_R2(D)                                                                    \
+  hipLaunchKernelGGL((k_filter_r2<MODE, D>), grid, dim3(kBS), 0, stream(), L.nelx, L.nely, \
+                     L.nelz, g.ewy, g.ewz, w[0], w[1], w[2], w[3], in, out)
+    if (d2max == 0) T3D_FILTER_R2(0);
+    if (d2max == 1) T3D_FILTER_R2(1);
+    if (d2max == 2) T3D_FILTER_R2(2);
+    if (d2max == 3) T3D_FILTER_R2(3);
+#undef T3D_FILTER_R2
+    return;
+  }
   FilterTaps taps;
   const int n = ne < (int64_t)INT32_MAX ? filter_taps(rmin, taps) : -1;
 #define T3D_FILTER_TAPS(NT)                                                                  \
@@ -468,7 +528,8 @@ int oc_update(const double *x, double *xnew, const double *dv, const double *dc,
   double *state = scratch;          // 8 doubles
   double *partials = scratch + 8;   // kOcCand * kPartials doubles
   const double init[8] = {0.0, 1e9, 0.0, 0.0, gval, 0, 0, 0};
-  hipMemcpyAsync(state, init, sizeof(init), hipMemcpyHostToDevice, stream());
+  if (!T3D_HIP(hipMemcpyAsync(state, init, sizeof(init), hipMemcpyHostToDevice, stream())))
+    return -1;
   double h[8];
   for (int it = 0; it < 4096; it += kOcChunk * kOcDepth) {
     for (int q = 0; q < kOcChunk; q++) {
@@ -479,8 +540,9 @@ int oc_update(const double *x, double *xnew, const double *dv, const double *dc,
     }
     hipLaunchKernelGGL(k_oc_write, dim3(kPartials), dim3(kBS), 0, stream(), nelem, x, xnew, dv,
                        dc, state);
-    hipMemcpyAsync(h, state, sizeof(h), hipMemcpyDeviceToHost, stream());
-    if (!T3D_HIP(hipStreamSynchronize(stream()))) return -1;
+    if (!T3D_HIP(hipMemcpyAsync(h, state, sizeof(h), hipMemcpyDeviceToHost, stream())) ||
+        !T3D_HIP(hipStreamSynchronize(stream())))
+      return -1;
     if (h[3] != 0.0) return (int)h[6];
   }
   return -1;
