@@ -201,6 +201,10 @@ void solveSubspaceMatrix(const struct gridContext gc, const int l,
  *      stencil_optimization.h:5-15) ---------------------------------------- */
 DTYPE compute_volume(const struct gridContext gc, DTYPE *xPhys);
 float compute_change(DTYPE *x, DTYPE *xnew, const int nelem);
+/* OC bisection (gpu_definitions.c:127-157) on the device, three bisection
+ * steps per pass over the elements; x_e sqrt(-dc_e / (dv_e lmid)) is formed
+ * as x_e sqrt(-dc_e / dv_e) / sqrt(lmid), within a few ulp of the one-step
+ * expression. */
 void update_solution(const DTYPE *x, DTYPE *xnew, const DTYPE *dv,
                      const DTYPE *dc, const int nelem, const DTYPE g);
 void enter_data(const struct gridContext gridContext,
