@@ -191,11 +191,12 @@ def test_sensitivity_filter_volume_change_oc(pair):
     assert torch.equal(xc, xnew)
 
 
-@pytest.mark.parametrize("rmin", [1.2, 2.0, 2.5, 4.5])
+@pytest.mark.parametrize("rmin", [1.2, 2.0, 2.2, 2.5, 4.5])
 def test_filter_radii(pair, rmin):
     """Filter and its transpose at radii with 7 and 27 non-zero window weights
-    (the fixed-window kernel for radii up to 2, as 1.5 in the other tests), 81
-    (the tap-list kernel) and more than 240 (the full-window kernel)."""
+    (the fixed-window kernel for radii up to 2, as 1.5 in the other tests), 33
+    and 81 (the tap-list kernels of 64 and 240 taps) and more than 240 (the
+    full-window kernel)."""
     g, o, xg, xo, rng = pair
     inter = g.interior_elements()
     assert relerr(g.filter(rmin, xg).cpu().numpy()[inter], o.filter(rmin, xo)[inter]) <= 1e-14

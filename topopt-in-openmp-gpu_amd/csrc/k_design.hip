@@ -204,7 +204,7 @@ __global__ __launch_bounds__(kBS) void k_filter(int nelx, int nely, int nelz, in
 // The non-zero weights of the window, in the loop order of the kernel above
 // (i2, then k2, then j2 ascending).  The window's zero weights add nothing to
 // either sum, so leaving them out keeps both sums bit for bit; the weights use
-// the same expression as the loop.  rmin 1.5: 19 of 64 terms.  The list is
+// the same expression as the loop (rmin 2.5: 81 of 216 terms).  The list is
 // padded to the kernel's unrolled length with zero weights at offset 0, which
 // add +0 to both sums.
 constexpr int kFilterMaxTaps = 240;
@@ -332,10 +332,7 @@ static void filter_pass(const DevGrid &g, double rmin, const double *in, double 
 #define T3D_FILTER_TAPS(NT)                                                                  \
   hipLaunchKernelGGL((k_filter_taps<MODE, NT>), grid, dim3(kBS), 0, stream(), L.nelx, L.nely, \
                      L.nelz, g.ewy, g.ewz, taps, in, out)
-  if (n >= 0 && n <= 8) T3D_FILTER_TAPS(8);
-  else if (n >= 0 && n <= 20) T3D_FILTER_TAPS(20);
-  else if (n >= 0 && n <= 28) T3D_FILTER_TAPS(28);
-  else if (n >= 0 && n <= 64) T3D_FILTER_TAPS(64);
+  if (n >= 0 && n <= 64) T3D_FILTER_TAPS(64);
   else if (n >= 0) T3D_FILTER_TAPS(kFilterMaxTaps);
   else
     hipLaunchKernelGGL((k_filter<MODE>), grid, dim3(kBS), 0, stream(), L.nelx, L.nely, L.nelz,
