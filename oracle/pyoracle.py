@@ -41,10 +41,16 @@ def build_if_missing():
 
 
 def load(typing="fp64"):
+    """The oracle library of one typing ("fp64" or "asship"), or the build at
+    the path `typing` when it names a .so file (the tests' deliberately wrong
+    copies of one function)."""
     if typing in _LIBS:
         return _LIBS[typing]
-    build_if_missing()
-    lib = ctypes.CDLL(os.path.join(_BUILD, f"liboracle_{typing}.so"))
+    if typing.endswith(".so"):
+        lib = ctypes.CDLL(typing)
+    else:
+        build_if_missing()
+        lib = ctypes.CDLL(os.path.join(_BUILD, f"liboracle_{typing}.so"))
     P = c_void_p
     sig = {
         "or_sizeof_ctype": (c_int, []),
@@ -238,6 +244,13 @@ class Grid:
         rt = c_double if self.dt == np.float64 else c_float
         self.lib.or_volume.restype = rt
         return float(self.lib.or_volume(self.h, _p(self._x(xphys))))
+
+    def change(self, x, xnew):
+        """(max |x - xnew| as float, x after the call: compute_change copies
+        xnew into x)."""
+        x = self._x(x).copy()
+        ch = self.lib.or_change(_p(x), _p(self._x(xnew)), x.size)
+        return ch, x
 
     def oc(self, x, dv, dc, g):
         x = self._x(x)

@@ -23,7 +23,14 @@
  *   or_sensitivity   c = u^T K u with the scipy operator; central FD of c(x)
  *                    with direct solves
  *   or_filter(_grad) explicit scipy filter matrix H; adjointness
- *   or_oc / or_volume / or_change  bisection invariants, move limits, bounds
+ *   or_oc            the OC optimality conditions: one multiplier for every
+ *                    unclamped element (fixes the exponent 1/2), clamped
+ *                    elements beyond their bound, volume balance to the
+ *                    bisection's resolution, a uniform closed form; copies
+ *                    built with a wrong move limit, exponent, bisection
+ *                    direction or a dropped volume term fail them
+ *   or_volume        numpy sum over the interior index set (halo ignored)
+ *   or_change        numpy max |x - xnew| in float, and the copy into x
  *   or_load          total force, node set, count
  *   or_top3d         composition of the above (trajectory fixtures are
  *                    regression data from this file, not independent)
@@ -1197,10 +1204,23 @@ float or_change(or_dtype *x, const or_dtype *xnew, int64_t nelem) {
 /* Optimality-criteria update by bisection on lambda:
  * update_solution, gpu_definitions.c:127-157.  The acceptance test mixes the
  * mean volume error g with the sum of dv*(xnew-x) over the padded array, as
- * the reference does.                                                       */
+ * the reference does.
+ *
+ * OR_OC_MOVE, OR_OC_ETA, OR_OC_FLIP and OR_OC_DROP_G exist only so that the
+ * tests can build deliberately wrong copies of this function (a wrong move
+ * limit, exponent, bisection direction, or a dropped volume term) and show
+ * that the OC checks reject each of them; the shipped builds never set them. */
+#ifndef OR_OC_MOVE
+#define OR_OC_MOVE 0.2
+#endif
+#ifdef OR_OC_ETA
+#define OR_OC_ROOT(v) pow((v), OR_OC_ETA)
+#else
+#define OR_OC_ROOT(v) sqrt(v)
+#endif
 void or_oc(const or_dtype *x, or_dtype *xnew, const or_dtype *dv,
            const or_dtype *dc, int64_t nelem, or_dtype g) {
-  or_dtype l1 = 0.0, l2 = 1e9, move = 0.2;
+  or_dtype l1 = 0.0, l2 = 1e9, move = OR_OC_MOVE;
   const int64_t nch = (nelem + OR_CHUNK - 1) / OR_CHUNK;
   or_dtype *part = (or_dtype *)malloc(sizeof(or_dtype) * (nch > 0 ? nch : 1));
   while ((l2 - l1) / (l1 + l2) > 1e-6) {
@@ -1213,14 +1233,20 @@ void or_oc(const or_dtype *x, or_dtype *xnew, const or_dtype *dv,
       for (int64_t i = ch * OR_CHUNK; i < i1; i++) {
         xnew[i] = OR_MAX(0.0, OR_MAX(x[i] - move,
                                      OR_MIN(1.0, OR_MIN(x[i] + move,
-                                                        x[i] * sqrt(-dc[i] / (dv[i] * lmid))))));
+                                                        x[i] * OR_OC_ROOT(-dc[i] / (dv[i] * lmid))))));
         ps += dv[i] * (xnew[i] - x[i]);
       }
       part[ch] = ps;
     }
     for (int64_t ch = 0; ch < nch; ch++) gt += part[ch];
+#ifndef OR_OC_DROP_G
     gt += g;
+#endif
+#ifdef OR_OC_FLIP
+    if (gt <= 0)
+#else
     if (gt > 0)
+#endif
       l1 = lmid;
     else
       l2 = lmid;

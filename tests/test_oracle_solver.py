@@ -190,3 +190,24 @@ def test_reference_bc_rows_stagnate_on_random_density():
     it, res = g.solve(x, g.load(), u, tol=1e-5, maxiter=200)
     assert it == 199 and res > 1e-3
     g.close()
+
+
+def test_asshipped_typing_c1_trajectory():
+    """The as-shipped typing (float CG/level vectors and densities,
+    definitions.h:25-29) on c1 (48x24x24, 3 levels, 5 design iterations):
+    its committed fixture is reproduced, it takes the FP64 typing's CG
+    iteration counts, and its compliances stay within the float vectors'
+    rounding (1e-4 relative) of the FP64 trajectory."""
+    import json
+    import os
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    ship = json.load(open(os.path.join(here, "c1_asship.json")))["records"]
+    fp64 = json.load(open(os.path.join(here, "c1_fp64.json")))["records"]
+    recs, _, _ = pyoracle.top3d(48, 24, 24, 3, 5, typing="asship")
+    assert len(recs) == len(ship) == len(fp64) == 5
+    for r, s, f in zip(recs, ship, fp64):
+        assert r["cgiter"] == s["cgiter"] == f["cgiter"]
+        assert r["c"] == pytest.approx(s["c"], rel=1e-6)
+        assert r["vol"] == pytest.approx(s["vol"], rel=1e-6)
+        assert r["c"] == pytest.approx(f["c"], rel=1e-4)
+        assert abs(r["c"] / f["c"] - 1) > 1e-9  # the typings do differ
