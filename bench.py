@@ -14,6 +14,10 @@ update, change, filter.  W untimed warm-up steps, then exactly K timed steps
 between device synchronisations.  Data are the synthetic cantilever (load and
 supports of the reference), uniform initial density 0.2.
 
+The line also carries the 512x256x256 run (coarse 16x8x8, 6 levels, mixed,
+OC fed by the autograd compliance; 5 design iterations after 1 warm-up):
+s/design-iter, DOFs/s, the live fine-sweep roofline and peak device memory.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
@@ -74,31 +78,123 @@ def measured_traffic(kernel, workload):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(nelx, nely, nelz, nl, gpu_iters):
-    """FP64 oracle on the host: per-solve set-up plus one CG iteration of the
-    first design iteration, extrapolated to the GPU's CG iteration count."""
-    import numpy as np
+def host_description():
+    """nproc, usable CPUs and the CPU model of this host (for the baseline)."""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"nproc": os.cpu_count(), "usable_cpus": usable, "cpu_model": model}
+
+
+def cpu_baseline(nelx, nely, nelz, nl, gpu_same):
+    """FP64 oracle on the host: the first design iteration of the cantilever
+    (uniform density 0.2, cold start; state solve with its set-up and CG,
+    volume, sensitivity, filter gradient, OC, change, filter), run whole and
+    timed, not extrapolated."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
-    g = pyoracle.Grid(nelx, nely, nelz, nl)
-    F = g.load()
-    _, wx, wy, wz, ndof = g.padding(0)
-    x = np.zeros(g.nelem())
-    i, k, j = np.meshgrid(np.arange(1, nelx + 1), np.arange(1, nelz + 1),
-                          np.arange(1, nely + 1), indexing="ij")
-    x[(i * (wy - 1) * (wz - 1) + k * (wy - 1) + j).ravel()] = float(np.float32(0.2))
-    U = np.zeros(ndof)
-    g.solve(x, F, U, maxiter=1)
-    setup_s, loop_s = g.solve_times()
+    host = host_description()
+    pyoracle.Grid(12, 6, 6, 3).close()  # load the library before the clock starts
+    try:
+        import ctypes
+        threads = ctypes.CDLL("libgomp.so.1").omp_get_max_threads()
+    except OSError:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or host["usable_cpus"]
+    t0 = time.perf_counter()
+    recs, _, _ = pyoracle.top3d(nelx, nely, nelz, nl, 1)
+    sec = time.perf_counter() - t0
+    return {"value": sec, "unit": "s/design-iter", "cores": threads, "kind": "port",
+            "sample": (f"FP64 oracle (oracle/top3d_oracle.c, OpenMP {threads} threads) running "
+                       f"the first design iteration of the same {nelx}x{nely}x{nelz} nl={nl} "
+                       f"cantilever whole (uniform density 0.2, cold start, "
+                       f"{recs[0]['cgiter'] + 1} CG iterations, then sensitivity, filters and "
+                       f"OC), timed end to end; the GPU takes "
+                       f"{gpu_same['mixed_s']:.4f} s (mixed) and {gpu_same['fp64_s']:.4f} s "
+                       f"(FP64) for that same design iteration"),
+            "host": host, "omp_num_threads": threads,
+            "compliance": recs[0]["c"], "cg_iters": recs[0]["cgiter"] + 1,
+            "gpu_same_iteration": gpu_same}
+
+
+def first_iteration_gpu(top3d_amd, torch, g):
+    """Whole first design iteration (uniform density, cold start) on a warm
+    grid: the work the CPU baseline times."""
+    st = top3d_amd.DesignState(g, volfrac=0.2, rmin=1.5)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rec = top3d_amd.design_step(st, cgtol=1e-5)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, rec
+
+
+def memory_record(top3d_amd, torch):
+    m = top3d_amd.memory_info()
+    return {"library_peak_GiB": m["library_peak_bytes"] / 2 ** 30,
+            "torch_peak_GiB": torch.cuda.max_memory_allocated() / 2 ** 30,
+            "device_used_GiB": m["device_used_bytes"] / 2 ** 30,
+            "device_total_GiB": m["device_total_bytes"] / 2 ** 30}
+
+
+def run_c5(top3d_amd, torch, cx, cy, cz, steps, warmup):
+    """Largest single-GPU run: coarse 16x8x8, 6 levels (512x256x256), mixed
+    form, OC fed by the autograd compliance and filter (design_step_autograd)."""
+    nl = 6
+    f = 1 << (nl - 1)
+    nelx, nely, nelz = cx * f, cy * f, cz * f
+    dofs = 3 * (nelx + 1) * (nely + 1) * (nelz + 1)
+    top3d_amd.memory_info(reset_peak=True)
+    torch.cuda.reset_peak_memory_stats()
+    g = top3d_amd.Grid(nelx, nely, nelz, nl, precision="mixed")
+    st = top3d_amd.DesignState(g, volfrac=0.2, rmin=1.5)
+    recs = []
+
+    def step():
+        t0 = time.perf_counter()
+        rec = top3d_amd.design_step_autograd(st, cgtol=1e-5)
+        torch.cuda.synchronize()
+        stt = g.stats()
+        rec["solve_s"] = (stt["solve_ms"] + stt["setup_ms"]) / 1e3
+        rec["step_s"] = time.perf_counter() - t0
+        return rec
+
+    for _ in range(warmup):
+        r = step()
+        print(f"[bench] c5 warm-up step: solve {r['solve_s']:.4f}s cg {r['cgiter'] + 1}",
+              file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    g.sweep_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        recs.append(step())
+        print(f"[bench] c5 step {len(recs)}: solve {recs[-1]['solve_s']:.4f}s "
+              f"cg {recs[-1]['cgiter'] + 1} c {recs[-1]['c']:.6e}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    sweep_ms, sweeps, sweep_bytes = g.sweep_timing(False)
+    solve_s = sum(r["solve_s"] for r in recs) / len(recs)
+    gbs = sweep_bytes / (sweep_ms * 1e-3) / 1e9 if sweep_ms > 0 else 0.0
+    out = {"workload": f"cantilever {nelx}x{nely}x{nelz}, coarse {cx}x{cy}x{cz}, {nl} levels, "
+                       f"mixed MGCG, OC fed by the autograd compliance/filter gradient",
+           "dofs": dofs, "steps": steps, "warmup": warmup,
+           "s_per_design_iter": wall / steps, "solve_s_per_design_iter": solve_s,
+           "dofs_per_s": dofs / solve_s, "cg_iters": [r["cgiter"] + 1 for r in recs],
+           "compliance": [r["c"] for r in recs],
+           "roofline": {"bound": "hbm", "kernel": "fine_sweep_f32", "achieved": gbs,
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                        "algorithmic_bytes": sweep_bytes, "avg_launch_ms": sweep_ms,
+                        "launches_timed": sweeps},
+           "memory": memory_record(top3d_amd, torch)}
     g.close()
-    value = setup_s + loop_s * gpu_iters
-    return {"value": value, "unit": "s/design-iter", "cores": threads, "kind": "port",
-            "sample": (f"FP64 oracle (oracle/top3d_oracle.c, OpenMP {threads} threads) on the same "
-                       f"{nelx}x{nely}x{nelz} nl={nl} cantilever from uniform density: per-solve "
-                       f"set-up {setup_s:.2f}s + 1 CG iteration {loop_s:.2f}s measured, "
-                       f"extrapolated to {gpu_iters:.1f} CG iterations per solve (the GPU FP64 "
-                       f"run's mean)")}
+    return out
 
 
 def main():
@@ -110,6 +206,9 @@ def main():
     ap.add_argument("--levels", type=int, default=5)
     ap.add_argument("--compare-fp64", type=int, default=1)
     ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--c5", type=int, default=1,
+                    help="also run the 512x256x256 (6 levels) mixed design loop with the "
+                         "autograd OC update for 5 design iterations")
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--kernel-only", default="",
                     help="profiling aid: one mixed design iteration, then only REPS launches of "
@@ -140,6 +239,8 @@ def main():
         g.close()
         return
 
+    torch.cuda.reset_peak_memory_stats()
+    top3d_amd.memory_info(reset_peak=True)
     g, st, recs, wall = run_form(top3d_amd, torch, nelx, nely, nelz, nl, "mixed",
                                  args.warmup, args.steps)
     solve_s = sum(r["solve_s"] for r in recs) / len(recs)
@@ -157,12 +258,14 @@ def main():
     dom = "fine_sweep_f32"
     traffic = measured_traffic(dom, f"{nelx}x{nely}x{nelz}")
     live_gbs = live["bytes"] / (live["avg_ms"] * 1e-3) / 1e9 if live["avg_ms"] > 0 else 0.0
-    g.close()
+    memory = memory_record(top3d_amd, torch)
 
     fp64 = None
+    contract = None
+    gpu_same = {}
     if args.compare_fp64:
-        g64, _, recs64, _ = run_form(top3d_amd, torch, nelx, nely, nelz, nl, "fp64",
-                                     args.warmup, args.steps)
+        g64, st64, recs64, _ = run_form(top3d_amd, torch, nelx, nely, nelz, nl, "fp64",
+                                        args.warmup, args.steps)
         s64 = sum(r["solve_s"] for r in recs64) / len(recs64)
         cdiff = max(abs(a["c"] - b["c"]) / abs(b["c"]) for a, b in zip(recs, recs64))
         fp64 = {"solve_s_per_design_iter": s64,
@@ -172,14 +275,39 @@ def main():
                 "mixed_vs_fp64_max_rel_compliance_diff": cdiff,
                 "fine_sweep_f64_live": recs64[0]["sweep_live"],
                 "speedup_mixed_over_fp64": s64 / solve_s}
+        # precision contract on one density: the FP64 run's last design,
+        # both forms from a cold start
+        xl = st64.xPhys
+        res = {}
+        for name, gg in (("fp64", g64), ("mixed", g)):
+            U = gg.zeros(0)
+            it, _ = gg.solve(xl, st64.F, U)
+            res[name] = (it + 1, float(st64.F @ U), U)
+        (i64, c64, u64), (i32, c32, u32) = res["fp64"], res["mixed"]
+        contract = {"density": f"last design of the FP64 run (design iteration "
+                               f"{args.warmup + args.steps}), cold start",
+                    "compliance_rel_err": abs(c32 - c64) / abs(c64),
+                    "displacement_rel_l2": float(torch.linalg.norm(u32 - u64) /
+                                                 torch.linalg.norm(u64)),
+                    "cg_iters_fp64": i64, "cg_iters_mixed": i32, "cg_iter_change": i32 - i64}
+        del u64, u32
+        if args.cpu_baseline:
+            gpu_same["mixed_s"], _ = first_iteration_gpu(top3d_amd, torch, g)
+            gpu_same["fp64_s"], _ = first_iteration_gpu(top3d_amd, torch, g64)
         g64.close()
+    g.close()
+
+    c5 = None
+    if args.c5:
+        print("[bench] 512x256x256, 6 levels, autograd OC ...", file=sys.stderr, flush=True)
+        c5 = run_c5(top3d_amd, torch, cx, cy, cz, 5, 1)
 
     cpu = None
     if args.cpu_baseline:
-        iters = (sum(fp64["cg_iters"]) / len(fp64["cg_iters"]) if fp64
-                 else sum(cg_mixed) / len(cg_mixed))
+        if not gpu_same:
+            gpu_same = {"mixed_s": float("nan"), "fp64_s": float("nan")}
         print("[bench] CPU baseline (oracle) ...", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(nelx, nely, nelz, nl, iters)
+        cpu = cpu_baseline(nelx, nely, nelz, nl, gpu_same)
 
     out = {
         "metric": "MGCG solve s/design-iter at 256x128x128, mixed vs FP64 @rtol 1e-5; matvec GB/s",
@@ -209,8 +337,11 @@ def main():
         "design_iter_s": wall / args.steps,
         "setup_s_per_design_iter": sum(r["setup_s"] for r in recs) / len(recs),
         "dofs_per_s": dofs / solve_s,
+        "memory": memory,
         "kernels": kernels,
         "fp64": fp64,
+        "precision_contract": contract,
+        "c5": c5,
     }
     print(json.dumps(out), flush=True)
 

@@ -300,6 +300,21 @@ int top3d_update_to(const void *host, uint64_t bytes);
 int top3d_update_from(void *host, uint64_t bytes);
 void *top3d_device_ptr(const void *host_or_device);
 
+/* Device memory: what the library holds (every allocation it makes: solver
+ * buffers, level data, coarse factor, present-table mirrors, scratch) now
+ * and at most since it was loaded or the peak was last reset, and the
+ * device-wide use from hipMemGetInfo (total - free: every allocator and
+ * process on the device).  reset_peak != 0 restarts the peak at the current
+ * holding after reading it.  No counterpart in the reference, whose device
+ * memory is OpenMP's present table.                                         */
+struct Top3dMemoryInfo {
+  uint64_t library_bytes;
+  uint64_t library_peak_bytes;
+  uint64_t device_used_bytes;
+  uint64_t device_total_bytes;
+};
+int top3d_memory_info(struct Top3dMemoryInfo *out, int reset_peak);
+
 /* stream on which all device work is ordered (0 = the library's own) */
 void top3d_set_stream(void *hip_stream);
 void *top3d_get_stream(void);

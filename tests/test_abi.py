@@ -83,3 +83,53 @@ def test_csr_pattern_matches_oracle():
     assert np.array_equal(ro, ref.indptr) and np.array_equal(ci, ref.indices)
     lib.freeSubspaceMatrix(ctypes.byref(M))
     o.close()
+
+
+# Layout of the reference's structs on x86-64 glibc, where uint_fast32_t is
+# 8 bytes: definitions.h:31-53, gpu_definitions.h:7-37 and
+# stencil_assembly.h:8-27 (CHOLMOD handles are pointers).
+REFERENCE_LAYOUT = {
+    "FixedDofs": 16, "FixedDofs.n": 0, "FixedDofs.idx": 8,
+    "gridContext": 120, "gridContext.E0": 0, "gridContext.Emin": 8, "gridContext.nu": 16,
+    "gridContext.elementSizeX": 24, "gridContext.elementSizeY": 32,
+    "gridContext.elementSizeZ": 40, "gridContext.nelx": 48, "gridContext.nely": 56,
+    "gridContext.nelz": 64, "gridContext.wrapx": 72, "gridContext.wrapy": 80,
+    "gridContext.wrapz": 88, "gridContext.penal": 96, "gridContext.precomputedKE": 104,
+    "gridContext.fixedDofs": 112,
+    "CSRMatrix": 40, "CSRMatrix.nnz": 0, "CSRMatrix.nrows": 8, "CSRMatrix.rowOffsets": 16,
+    "CSRMatrix.colIndex": 24, "CSRMatrix.vals": 32,
+    "CoarseSolverData": 56, "CoarseSolverData.cholmodCommon": 0,
+    "CoarseSolverData.factoredMatrix": 16, "CoarseSolverData.E_workspace": 48,
+    "SolverDataBuffer": 128, "SolverDataBuffer.r": 0, "SolverDataBuffer.p": 8,
+    "SolverDataBuffer.q": 16, "SolverDataBuffer.z": 24, "SolverDataBuffer.invD": 32,
+    "SolverDataBuffer.dmg": 40, "SolverDataBuffer.rmg": 48, "SolverDataBuffer.zmg": 56,
+    "SolverDataBuffer.coarseMatrices": 64, "SolverDataBuffer.bottomSolver": 72,
+    "gpuNode": 40, "gpuNode.id": 0, "gpuNode.prev": 8, "gpuNode.next": 16, "gpuNode.gc": 24,
+    "gpuNode.offset_x": 32, "gpuNode.x_global": 36,
+    "gpuGrid": 16, "gpuGrid.num_targets": 0, "gpuGrid.target": 8,
+    "uint_fast32_t": 8,
+}
+
+
+def test_header_compiled_as_c_has_reference_layout(tmp_path):
+    """include/top3d_hip.h compiled by a C compiler (as the reference's
+    callers are) lays out every struct as the reference does, and the
+    Python binding's ctypes mirror agrees."""
+    import subprocess
+    exe = tmp_path / "abi_layout"
+    subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "native", "abi_layout.c"), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    got = {k: int(v) for k, v in (line.split() for line in out.splitlines())}
+    assert got == REFERENCE_LAYOUT
+    mirror = {"FixedDofs": top3d_amd.FixedDofs, "gridContext": top3d_amd.GridContext,
+              "CSRMatrix": top3d_amd.CSRMatrix, "CoarseSolverData": top3d_amd.CoarseSolverData,
+              "SolverDataBuffer": top3d_amd.SolverDataBuffer, "gpuNode": top3d_amd.GpuNode,
+              "gpuGrid": top3d_amd.GpuGrid}
+    for key, val in got.items():
+        if key == "uint_fast32_t":
+            continue
+        s, _, field = key.partition(".")
+        cls = mirror[s]
+        assert (getattr(cls, field).offset if field else ctypes.sizeof(cls)) == val, key

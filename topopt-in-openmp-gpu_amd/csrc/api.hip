@@ -652,9 +652,12 @@ void update_solution(const DTYPE *x, DTYPE *xnew, const DTYPE *dv, const DTYPE *
       !vc.open(dc, sizeof(double) * nelem, true, false))
     return;
   double *scratch = (double *)dalloc(sizeof(double) * kOcScratchDoubles);
-  oc_update((const double *)vx.dev, (double *)vn.dev, (const double *)vv.dev,
-            (const double *)vc.dev, nelem, gval, scratch);
-  dfree(scratch);
+  if (scratch) {
+    // a failure is recorded by oc_update (HIP error, or no convergence)
+    oc_update((const double *)vx.dev, (double *)vn.dev, (const double *)vv.dev,
+              (const double *)vc.dev, nelem, gval, scratch);
+    dfree(scratch);
+  }
   vc.close();
   vv.close();
   vn.close();

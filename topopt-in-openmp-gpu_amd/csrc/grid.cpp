@@ -64,16 +64,48 @@ hipStream_t stream() {
   return g_stream;
 }
 
-void *dalloc(size_t bytes) {
+// Device memory held by the library (every allocation goes through
+// dalloc/dfree), for top3d_memory_info.
+namespace {
+std::mutex g_amu;
+std::map<uintptr_t, size_t> g_alloc;
+uint64_t g_held = 0, g_peak = 0;
+}  // namespace
+
+static void track_alloc(void *p, size_t bytes) {
+  std::lock_guard<std::mutex> lk(g_amu);
+  g_alloc[(uintptr_t)p] = bytes;
+  g_held += bytes;
+  if (g_held > g_peak) g_peak = g_held;
+}
+
+static void untrack_alloc(void *p) {
+  std::lock_guard<std::mutex> lk(g_amu);
+  auto it = g_alloc.find((uintptr_t)p);
+  if (it == g_alloc.end()) return;
+  g_held -= it->second;
+  g_alloc.erase(it);
+}
+
+void *dalloc_uninit(size_t bytes) {
   void *p = nullptr;
   if (bytes == 0) bytes = 8;
   if (!hip_ok(hipMalloc(&p, bytes), "hipMalloc")) return nullptr;
-  if (!hip_ok(hipMemsetAsync(p, 0, bytes, stream()), "hipMemsetAsync")) return nullptr;
+  track_alloc(p, bytes);
+  return p;
+}
+
+void *dalloc(size_t bytes) {
+  void *p = dalloc_uninit(bytes);
+  if (!p) return nullptr;
+  if (!hip_ok(hipMemsetAsync(p, 0, bytes ? bytes : 8, stream()), "hipMemsetAsync")) return nullptr;
   return p;
 }
 
 void dfree(void *p) {
-  if (p) hip_ok(hipFree(p), "hipFree");
+  if (!p) return;
+  untrack_alloc(p);
+  hip_ok(hipFree(p), "hipFree");
 }
 
 // ---------------------------------------------------------------------------
@@ -346,18 +378,36 @@ static int map_impl(const void *host, uint64_t bytes, bool copy) {
     }
     return TOP3D_OK;
   }
-  void *d = nullptr;
-  if (!T3D_HIP(hipMalloc(&d, bytes))) return TOP3D_ERR_HIP;
+  void *d = dalloc_uninit(bytes);
+  if (!d) return TOP3D_ERR_HIP;
   if (copy) {
     if (!T3D_HIP(hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, stream())) ||
         !T3D_HIP(hipStreamSynchronize(stream()))) {
-      hipFree(d);
+      dfree(d);
       return TOP3D_ERR_HIP;
     }
   } else {
     T3D_HIP(hipMemsetAsync(d, 0, bytes, stream()));
   }
   g_present[(uintptr_t)host] = Mapping{d, bytes};
+  return TOP3D_OK;
+}
+
+int top3d_memory_info(struct Top3dMemoryInfo *out, int reset_peak) {
+  if (!out) {
+    set_error(TOP3D_ERR_ARG, "top3d_memory_info: null output");
+    return TOP3D_ERR_ARG;
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_amu);
+    out->library_bytes = g_held;
+    out->library_peak_bytes = g_peak;
+    if (reset_peak) g_peak = g_held;
+  }
+  size_t fr = 0, tot = 0;
+  if (!T3D_HIP(hipMemGetInfo(&fr, &tot))) return TOP3D_ERR_HIP;
+  out->device_used_bytes = (uint64_t)(tot - fr);
+  out->device_total_bytes = (uint64_t)tot;
   return TOP3D_OK;
 }
 
@@ -372,7 +422,7 @@ int top3d_unmap(const void *host) {
     return TOP3D_ERR_UNMAPPED;
   }
   hipStreamSynchronize(stream());
-  hipFree(it->second.dev);
+  dfree(it->second.dev);
   g_present.erase(it);
   return TOP3D_OK;
 }

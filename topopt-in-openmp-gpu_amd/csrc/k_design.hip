@@ -516,17 +516,26 @@ __global__ __launch_bounds__(kBS) void k_oc_write(int64_t n, const double *__res
   }
 }
 
+// the bisection state of one OC update: [l1, l2] = [0, 1e9], not converged
+__global__ void k_oc_init(double gval, double *__restrict__ state) {
+  if (threadIdx.x < 8) {
+    const double init[8] = {0.0, 1e9, 0.0, 0.0, gval, 0.0, 0.0, 0.0};
+    state[threadIdx.x] = init[threadIdx.x];
+  }
+}
+
 // Runs the bisection, checking on the host every kOcChunk passes (passes
 // after convergence return at once).  xnew is written after each chunk for
 // the last multiplier, so a converged chunk needs no further launch.
-// Returns the number of bisection steps or -1.
+// scratch (kOcScratchDoubles doubles of device memory) is initialised here:
+// its previous contents do not matter.  Returns the number of bisection
+// steps or -1.
 int oc_update(const double *x, double *xnew, const double *dv, const double *dc,
               int64_t nelem, double gval, double *scratch) {
   double *state = scratch;          // 8 doubles
   double *partials = scratch + 8;   // kOcCand * kPartials doubles
-  const double init[8] = {0.0, 1e9, 0.0, 0.0, gval, 0, 0, 0};
-  if (!T3D_HIP(hipMemcpyAsync(state, init, sizeof(init), hipMemcpyHostToDevice, stream())))
-    return -1;
+  hipLaunchKernelGGL(k_oc_init, dim3(1), dim3(64), 0, stream(), gval, state);
+  if (!T3D_HIP(hipGetLastError())) return -1;
   double h[8];
   for (int it = 0; it < 4096; it += kOcChunk * kOcDepth) {
     for (int q = 0; q < kOcChunk; q++) {
@@ -542,6 +551,7 @@ int oc_update(const double *x, double *xnew, const double *dv, const double *dc,
       return -1;
     if (h[3] != 0.0) return (int)h[6];
   }
+  set_error(TOP3D_ERR_NO_CONVERGE, "update_solution: OC bisection did not converge in 4096 steps");
   return -1;
 }
 

@@ -67,7 +67,7 @@ bool SolverState::allocate(DevGrid *grid, int nlev) {
   }
   // level-0 V-cycle vectors in FP64 alias the CG vectors r (input) and z
   v64[0][kR] = r;
-  hipFree(v64[0][kZ]);
+  dfree(v64[0][kZ]);
   v64[0][kZ] = z;
   const Level &C = g->lv[nl - 1];
   ncoarse = (int)(3 * C.interior_nodes());
@@ -92,7 +92,7 @@ bool SolverState::allocate(DevGrid *grid, int nlev) {
 void SolverState::release() {
   hipStreamSynchronize(stream());
   if (aux) hipStreamSynchronize(aux);
-  auto f = [](void *ptr) { if (ptr) hipFree(ptr); };
+  auto f = [](void *ptr) { dfree(ptr); };
   for (int l = 0; l < nl; l++) {
     f(invD64[l]);
     f(invD32[l]);

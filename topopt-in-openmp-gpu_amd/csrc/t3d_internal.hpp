@@ -91,7 +91,8 @@ void *dptr(const void *p, const char *what);  // mapped host or device pointer
 bool is_device_ptr(const void *p);
 
 // device allocation helpers (zero-filled)
-void *dalloc(size_t bytes);
+void *dalloc(size_t bytes);          // zero-filled on the library stream
+void *dalloc_uninit(size_t bytes);   // contents undefined
 void dfree(void *p);
 
 // ---------------------------------------------------------------------------

@@ -76,6 +76,20 @@ class GpuGrid(ctypes.Structure):
     _fields_ = [("num_targets", c_int), ("target", ctypes.POINTER(GpuNode))]
 
 
+class MemoryInfo(ctypes.Structure):
+    _fields_ = [("library_bytes", c_u64), ("library_peak_bytes", c_u64),
+                ("device_used_bytes", c_u64), ("device_total_bytes", c_u64)]
+
+
+def memory_info(reset_peak=False):
+    """Device memory of the library (now and peak) and of the whole device
+    (top3d_memory_info), in bytes."""
+    m = MemoryInfo()
+    if library().top3d_memory_info(ctypes.byref(m), int(bool(reset_peak))):
+        check("top3d_memory_info")
+    return {f: getattr(m, f) for f, _ in MemoryInfo._fields_}
+
+
 class SolveStats(ctypes.Structure):
     _fields_ = [("iterations", c_int), ("relres", c_double), ("setup_ms", c_double),
                 ("solve_ms", c_double), ("status", c_int)]
@@ -153,6 +167,7 @@ SIGNATURES = {
     "top3d_compliance_backward": (c_int, [P, c_double, P, P, c_double, P]),
     "top3d_dense_spd_inverse": (c_int, [P, c_int, c_int]),
     "top3d_time_kernel": (c_int, [P, c_int, c_int, P, P]),
+    "top3d_memory_info": (c_int, [P, c_int]),
     "top3d_sweep_timing": (c_int, [P, c_int, P, P, P]),
     "applyStateOperator_stencil_f32": (None, [P, P, P, P]),
     "applyStateOperatorSubspace_halo_f32": (None, [P, c_int, P, P, P]),
@@ -278,6 +293,11 @@ class Grid:
     def fixed_dofs(self, l):
         fd = self.gc.fixedDofs[l]
         return np.ctypeslib.as_array(fd.idx, (fd.n,)).astype(np.int64).copy()
+
+    def _interior_index(self):
+        if getattr(self, "_iidx", None) is None:
+            self._iidx = self.torch.from_numpy(self.interior_elements()).to(self.device)
+        return self._iidx
 
     def interior_elements(self):
         ey, ez = self.gc.wrapy - 1, self.gc.wrapz - 1
@@ -492,40 +512,145 @@ def design_loop(grid, iters, volfrac=0.2, rmin=1.5, cgtol=1e-5):
     return recs, st
 
 
+def _compliance_fn():
+    import torch
+
+    class _Compliance(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, xp, grid, F, U, tol, allow_no_converge):
+            U = grid.zeros(0) if U is None else U
+            c = c_double()
+            grid._sync_torch()
+            grid.lib.top3d_clear_error()
+            rc = grid.lib.top3d_compliance_forward(ctypes.byref(grid.gpu), ctypes.byref(grid.data),
+                                                   grid.nl, tol, _ptr(xp), _ptr(F), _ptr(U),
+                                                   ctypes.byref(c))
+            st = grid.stats()
+            if rc == 5 and not allow_no_converge:
+                check("top3d_compliance_forward")
+            if rc not in (0, 5):
+                check("top3d_compliance_forward")
+            grid.lib.top3d_clear_error()
+            if rc == 5:
+                import warnings
+                warnings.warn(f"Compliance: CG stopped at {st['iterations']} iterations with "
+                              f"relative residual {st['relres']:.3e} above {tol:.1e}",
+                              RuntimeWarning, stacklevel=3)
+            ctx.iterations, ctx.relres, ctx.converged = st["iterations"], st["relres"], rc == 0
+            grid.last_compliance_solve = dict(iterations=st["iterations"], relres=st["relres"],
+                                              converged=rc == 0)
+            ctx.grid = grid
+            ctx.save_for_backward(xp, U.clone())
+            return torch.tensor(c.value, dtype=torch.float64, device=xp.device)
+
+        @staticmethod
+        def backward(ctx, gout):
+            xp, U = ctx.saved_tensors
+            grid = ctx.grid
+            grad = torch.zeros_like(xp)
+            grid._sync_torch()
+            rc = grid.lib.top3d_compliance_backward(ctypes.byref(grid.gpu), 0.0, _ptr(xp),
+                                                    _ptr(U), float(gout), _ptr(grad))
+            if rc:
+                check("top3d_compliance_backward")
+            return grad, None, None, None, None, None
+
+    return _Compliance
+
+
+def _filter_fn():
+    import torch
+
+    class _DensityFilter(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, x, grid, rmin):
+            ctx.grid, ctx.rmin = grid, rmin
+            return grid.filter(rmin, x.detach())
+
+        @staticmethod
+        def backward(ctx, gout):
+            grid = ctx.grid
+            g = torch.zeros_like(gout)
+            idx = grid._interior_index()
+            g[idx] = gout[idx]
+            out = torch.zeros_like(gout)
+            out[idx] = grid.filter_grad(ctx.rmin, g)[idx]
+            return out, None, None
+
+    return _DensityFilter
+
+
+_FNS = {}
+
+
 class Compliance:
-    """torch.autograd.Function c(xPhys) = F^T u(xPhys) with the self-adjoint
-    backward dc/dx = -3 (E0-Emin) x^2 u_e^T KE u_e (no second solve),
-    optionally pulled back through the density filter (rmin > 0)."""
+    """torch.autograd.Function c(xPhys) = sum_e E(xPhys_e) u_e^T KE u_e with u
+    from the MGCG state solve (solveStateMG_halo, stencil_solvers.c:259-397)
+    and c from getComplianceAndSensetivity_halo (stencil_methods.c:808-862).
+
+    backward is the self-adjoint sensitivity dc/dxPhys_e = -3 (E0 - Emin)
+    xPhys_e^2 u_e^T KE u_e from the forward displacement (no second solve),
+    scaled by the incoming gradient.  It is the gradient with respect to
+    xPhys only; compose with DensityFilter for the gradient with respect to
+    the design variables (stencil_optimization.c:160-162).
+
+    U: optional float64 displacement tensor used as the warm start and
+    overwritten with the solution (the reference carries U from one design
+    iteration to the next).  A solve that stops at maxIter above `tol`
+    raises Top3dError unless allow_no_converge is set, in which case it
+    warns; the iteration count and relative residual are kept on
+    grid.last_compliance_solve either way."""
 
     @staticmethod
-    def apply(xphys, grid, F, tol=1e-5, rmin=0.0):
-        import torch
+    def apply(xphys, grid, F, U=None, tol=1e-5, allow_no_converge=False):
+        if "c" not in _FNS:
+            _FNS["c"] = _compliance_fn()
+        return _FNS["c"].apply(xphys, grid, F, U, float(tol), bool(allow_no_converge))
 
-        class _Fn(torch.autograd.Function):
-            @staticmethod
-            def forward(ctx, xp):
-                U = grid.zeros(0)
-                c = c_double()
-                grid._sync_torch()
-                rc = grid.lib.top3d_compliance_forward(ctypes.byref(grid.gpu),
-                                                       ctypes.byref(grid.data), grid.nl, tol,
-                                                       _ptr(xp), _ptr(F), _ptr(U),
-                                                       ctypes.byref(c))
-                if rc not in (0, 5):
-                    check("top3d_compliance_forward")
-                grid.lib.top3d_clear_error()
-                ctx.save_for_backward(xp, U)
-                return torch.tensor(c.value, dtype=torch.float64, device=xp.device)
 
-            @staticmethod
-            def backward(ctx, gout):
-                xp, U = ctx.saved_tensors
-                grad = torch.zeros_like(xp)
-                grid._sync_torch()
-                rc = grid.lib.top3d_compliance_backward(ctypes.byref(grid.gpu), rmin, _ptr(xp),
-                                                        _ptr(U), float(gout), _ptr(grad))
-                if rc:
-                    check("top3d_compliance_backward")
-                return grad
+class DensityFilter:
+    """torch.autograd.Function xPhys = (H x) / Hs over the interior elements
+    (applyDensityFilter, stencil_optimization.c:228-285); backward pulls a
+    gradient back as H (g / Hs) (applyDensityFilterGradient,
+    stencil_optimization.c:290-395).  Halo elements get zero gradient."""
 
-        return _Fn.apply(xphys)
+    @staticmethod
+    def apply(x, grid, rmin):
+        if "f" not in _FNS:
+            _FNS["f"] = _filter_fn()
+        return _FNS["f"].apply(x, grid, float(rmin))
+
+
+def design_step_autograd(st, cgtol=1e-5, allow_no_converge=False):
+    """One design iteration of top3dmgcg (stencil_optimization.c:150-189) in
+    which torch autograd supplies the OC update's sensitivities: xPhys =
+    DensityFilter(x), c = Compliance(xPhys), c.backward() gives
+    dc/dx = H (dc/dxPhys / Hs), the filtered sensitivity of
+    stencil_optimization.c:160-162.  The state U is carried over as warm
+    start, as in the reference."""
+    g = st.grid
+    st.loop += 1
+    xl = st.x.detach().requires_grad_(True)
+    xphys = DensityFilter.apply(xl, g, st.rmin)
+    if st.loop == 1:
+        # the reference's first iteration uses xPhys = x (stencil_optimization.c:104-120)
+        xphys = xphys + (st.xPhys - xphys).detach()
+    c = Compliance.apply(xphys, g, st.F, st.U, float(np.float32(cgtol)), allow_no_converge)
+    c.backward()
+    info = g.last_compliance_solve
+    st.xPhys = xphys.detach()
+    vol = g.volume(st.xPhys) / (g.nelx * g.nely * g.nelz)
+    st.dc = xl.grad
+    g.oc(st.x, st.xnew, st.dv, st.dc, vol - st.volfrac)
+    st.change = g.change(st.x, st.xnew)
+    return dict(loop=st.loop, c=float(c), vol=vol, change=st.change, relres=info["relres"],
+                cgiter=info["iterations"] - 1)
+
+
+def design_loop_autograd(grid, iters, volfrac=0.2, rmin=1.5, cgtol=1e-5):
+    st = DesignState(grid, volfrac, rmin)
+    recs = []
+    while st.change > 1e-2 and st.loop < iters:
+        recs.append(design_step_autograd(st, cgtol))
+    st.xPhys = grid.filter(st.rmin, st.x)
+    return recs, st
