@@ -103,7 +103,7 @@ def cpu_baseline(nelx, nely, nelz, nl, gpu_same):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     host = host_description()
-    pyoracle.Grid(12, 6, 6, 3).close()  # load the library before the clock starts
+    pyoracle.Grid(16, 8, 8, 3).close()  # load the library before the clock starts
     try:
         import ctypes
         threads = ctypes.CDLL("libgomp.so.1").omp_get_max_threads()

@@ -643,7 +643,7 @@ def design_step_autograd(st, cgtol=1e-5, allow_no_converge=False):
     st.dc = xl.grad
     g.oc(st.x, st.xnew, st.dv, st.dc, vol - st.volfrac)
     st.change = g.change(st.x, st.xnew)
-    return dict(loop=st.loop, c=float(c), vol=vol, change=st.change, relres=info["relres"],
+    return dict(loop=st.loop, c=float(c.detach()), vol=vol, change=st.change, relres=info["relres"],
                 cgiter=info["iterations"] - 1)
 
 
