@@ -207,6 +207,19 @@ float compute_change(DTYPE *x, DTYPE *xnew, const int nelem);
  * expression. */
 void update_solution(const DTYPE *x, DTYPE *xnew, const DTYPE *dv,
                      const DTYPE *dc, const int nelem, const DTYPE g);
+/* update_solution for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same arguments and result; x, xnew, dv
+ * and dc must be device pointers or mapped host arrays (unmapped host arrays
+ * are rejected, TOP3D_ERR_UNMAPPED, instead of being staged).  It calls no
+ * hipMalloc/hipFree: the bisection state and the per-workgroup partial sums
+ * live in a per-device static array of 8 + 15 * 1024 doubles (123 KB, the
+ * same for every nelem; nelem < 0 is rejected), held until the process
+ * ends.  Calls on one GPU share that array, so calls that overlap on one GPU
+ * (two host threads, or two streams) race; each call initialises the array
+ * itself.                                                                    */
+void update_solution_noalloc(const DTYPE *x, DTYPE *xnew, const DTYPE *dv,
+                             const DTYPE *dc, const int nelem, const DTYPE g);
 void enter_data(const struct gridContext gridContext,
                 const struct SolverDataBuffer solverData,
                 const struct gpuGrid gpu_gc, const DTYPE *xPhys,

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "solver_state.hpp"
@@ -662,6 +663,54 @@ void update_solution(const DTYPE *x, DTYPE *xnew, const DTYPE *dv, const DTYPE *
   vv.close();
   vn.close();
   vx.close();
+}
+
+// ---- OC update without allocation -------------------------------------------
+// update_solution_noalloc keeps the bisection state and the per-workgroup
+// partial sums in this file-scope array: one copy per device, fixed size
+// whatever nelem is, held until the process ends.  Host code never passes the
+// array's name to a kernel (on the host it is only a stand-in for the symbol);
+// it uses the device address from hipGetSymbolAddress, looked up once per
+// device.
+static __device__ __align__(256) double scratch[kOcScratchDoubles];
+constexpr int kNoAllocMaxDevices = 64;
+static double *g_scratch_dev[kNoAllocMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+static double *noalloc_scratch() {
+  int dev = -1;
+  if (!T3D_HIP(hipGetDevice(&dev))) return nullptr;
+  if (dev < 0 || dev >= kNoAllocMaxDevices) {
+    set_error(TOP3D_ERR_ARG, "update_solution_noalloc: device %d beyond %d", dev,
+              kNoAllocMaxDevices);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    if (!T3D_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)))) return nullptr;
+    g_scratch_dev[dev] = (double *)p;
+  }
+  return g_scratch_dev[dev];
+}
+
+// gpu_definitions.c:127-157 as update_solution, allocating nothing: the
+// arrays must be device pointers or mapped host arrays (no staging), and the
+// scratch is the per-device array above.
+void update_solution_noalloc(const DTYPE *x, DTYPE *xnew, const DTYPE *dv, const DTYPE *dc,
+                             const int nelem, const DTYPE gval) {
+  if (nelem < 0) {
+    set_error(TOP3D_ERR_ARG, "update_solution_noalloc: nelem %d < 0", nelem);
+    return;
+  }
+  const double *dx = (const double *)dptr(x, "update_solution_noalloc x");
+  double *dn = (double *)dptr(xnew, "update_solution_noalloc xnew");
+  const double *ddv = (const double *)dptr(dv, "update_solution_noalloc dv");
+  const double *ddc = (const double *)dptr(dc, "update_solution_noalloc dc");
+  if (!dx || !dn || !ddv || !ddc) return;
+  double *s = noalloc_scratch();
+  if (!s) return;
+  oc_update(dx, dn, ddv, ddc, nelem, gval, s);
 }
 
 // stencil_optimization.c:228-285 (runs on the GPU; host arrays are staged)

@@ -132,6 +132,7 @@ SIGNATURES = {
     "compute_volume": (c_double, [GridContext, P]),
     "compute_change": (c_float, [P, P, c_int]),
     "update_solution": (None, [P, P, P, P, c_int, c_double]),
+    "update_solution_noalloc": (None, [P, P, P, P, c_int, c_double]),
     "enter_data": (None, [GridContext, SolverDataBuffer, GpuGrid, P, P, P, P, P, P, c_int,
                           c_int]),
     "applyDensityFilter": (None, [GridContext, c_double, P, P]),
