@@ -102,7 +102,7 @@ def test_c5_autograd_gradient_directional_fd():
     xl = x.clone().requires_grad_(True)
     c = top3d_amd.Compliance.apply(top3d_amd.DensityFilter.apply(xl, g, rmin), g, F, tol=tol)
     c.backward()
-    print(f"[gpu c5] c = {float(c):.12e}, CG {g.last_compliance_solve['iterations']}",
+    print(f"[gpu c5] c = {float(c.detach()):.12e}, CG {g.last_compliance_solve['iterations']}",
           flush=True)
 
     def c_cold(xx):

@@ -62,100 +62,129 @@ void top3d_line_load_profile(const struct gridContext *gc, CTYPE *F, int profile
   }
 }
 
+// The design arrays of the loop, host arrays in the padded element layout
+// (mapped to the device by enter_data), freed together.
+struct DesignArrays {
+  uint64_t count = 0;
+  DTYPE *sens = nullptr;     // compliance sensitivity, then its filtered form
+  DTYPE *vol_sens = nullptr; // volume sensitivity through the filter
+  DTYPE *design = nullptr;   // design variables
+  DTYPE *phys = nullptr;     // filtered densities the solve sees
+  DTYPE *next = nullptr;     // the OC update's proposal
+  bool allocate(uint64_t n, DTYPE fill_value, const struct gridContext &grid) {
+    count = n;
+    sens = (DTYPE *)std::calloc(n, sizeof(DTYPE));
+    vol_sens = (DTYPE *)std::malloc(sizeof(DTYPE) * n);
+    design = (DTYPE *)std::calloc(n, sizeof(DTYPE));
+    phys = (DTYPE *)std::calloc(n, sizeof(DTYPE));
+    next = (DTYPE *)std::calloc(n, sizeof(DTYPE));
+    if (!sens || !vol_sens || !design || !phys || !next) return false;
+    for (uint64_t e = 0; e < n; e++) vol_sens[e] = 1.0;
+    // uniform start on the real elements, the halo at 0 (stencil_optimization.c:103-119)
+    const uint64_t sy = grid.wrapy - 1, sz = grid.wrapz - 1;
+    for (uint64_t i = 1; i <= grid.nelx; i++)
+      for (uint64_t k = 1; k <= grid.nelz; k++)
+        for (uint64_t j = 1; j <= grid.nely; j++) {
+          design[i * sy * sz + k * sy + j] = fill_value;
+          phys[i * sy * sz + k * sy + j] = fill_value;
+        }
+    return true;
+  }
+  void release() {
+    for (DTYPE *p : {sens, vol_sens, design, phys, next}) std::free(p);
+  }
+};
+
 void top3dmgcg(const uint_fast32_t nelx, const uint_fast32_t nely, const uint_fast32_t nelz,
                const DTYPE volfrac, const DTYPE rmin, const uint_fast32_t nl, const float cgtol,
                const uint_fast32_t cgmax, const int verbose, const int write_result,
                const int max_iterations) {
-  (void)cgmax;  // passed but unused by the reference too
-  struct gridContext gridContext;
-  setupGC(&gridContext, (int)nl, (int)nelx, (int)nely, (int)nelz);
+  (void)cgmax;  // the reference passes it and never reads it either
+  const int levels = (int)nl;
+  struct gridContext grid;
+  setupGC(&grid, levels, (int)nelx, (int)nely, (int)nelz);
   if (top3d_last_error()) {
     std::fprintf(stderr, "top3d: %s\n", top3d_last_error_string());
     return;
   }
-  struct gpuGrid gpu_gc;
-  gpu_gc.num_targets = 1;
-  gridContextToGPUGrid(&gridContext, &gpu_gc, (int)nl, verbose);
-  const uint64_t nelem = (uint64_t)(gridContext.wrapx - 1) * (gridContext.wrapy - 1) *
-                         (gridContext.wrapz - 1);
-  CTYPE *F;
-  STYPE *U;
-  allocateZeroPaddedStateField(gridContext, 0, &F);
-  allocateZeroPaddedStateField_STYPE(gridContext, 0, &U);
-  top3d_cantilever_load(&gridContext, F);
-  DTYPE *dc = (DTYPE *)std::calloc(nelem, sizeof(DTYPE));
-  DTYPE *dv = (DTYPE *)std::malloc(sizeof(DTYPE) * nelem);
-  DTYPE *x = (DTYPE *)std::calloc(nelem, sizeof(DTYPE));
-  DTYPE *xPhys = (DTYPE *)std::calloc(nelem, sizeof(DTYPE));
-  DTYPE *xnew = (DTYPE *)std::calloc(nelem, sizeof(DTYPE));
-  DTYPE c = 0.0;
-  for (uint64_t i = 0; i < nelem; i++) dv[i] = 1.0;
-  for (uint64_t i = 1; i < gridContext.nelx + 1; i++)
-    for (uint64_t k = 1; k < gridContext.nelz + 1; k++)
-      for (uint64_t j = 1; j < gridContext.nely + 1; j++) {
-        const uint64_t idx = i * (gridContext.wrapy - 1) * (gridContext.wrapz - 1) +
-                             k * (gridContext.wrapy - 1) + j;
-        x[idx] = volfrac;
-        xPhys[idx] = volfrac;
-      }
-  struct SolverDataBuffer solverData;
-  allocateSolverData(gridContext, (int)nl, &solverData);
-  top3d_set_precision(&solverData, g_default_precision);
-  top3d_set_bc_mode(&solverData, g_default_bc);
-  top3d_map(dv, sizeof(DTYPE) * nelem);
-  applyDensityFilterGradient(gridContext, rmin, dv);
-  const auto start = std::chrono::steady_clock::now();
-  auto wtime = [&]() {
-    return std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+  struct gpuGrid devices;
+  devices.num_targets = 1;
+  gridContextToGPUGrid(&grid, &devices, levels, verbose);
+  const uint64_t n_elem = (uint64_t)(grid.wrapx - 1) * (grid.wrapy - 1) * (grid.wrapz - 1);
+  CTYPE *load;
+  STYPE *disp;
+  allocateZeroPaddedStateField(grid, 0, &load);
+  allocateZeroPaddedStateField_STYPE(grid, 0, &disp);
+  top3d_cantilever_load(&grid, load);
+  DesignArrays arr;
+  if (!arr.allocate(n_elem, volfrac, grid)) {
+    std::fprintf(stderr, "top3d: out of host memory\n");
+    arr.release();
+    return;
+  }
+  struct SolverDataBuffer solver;
+  allocateSolverData(grid, levels, &solver);
+  top3d_set_precision(&solver, g_default_precision);
+  top3d_set_bc_mode(&solver, g_default_bc);
+  // the volume sensitivity is filtered once, before the loop
+  top3d_map(arr.vol_sens, sizeof(DTYPE) * n_elem);
+  applyDensityFilterGradient(grid, rmin, arr.vol_sens);
+  const auto t_start = std::chrono::steady_clock::now();
+  auto seconds = [&]() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
   };
-  enter_data(gridContext, solverData, gpu_gc, xPhys, x, xnew, U, F, dc, (int)nelem, (int)nl);
+  enter_data(grid, solver, devices, arr.phys, arr.design, arr.next, disp, load, arr.sens,
+             (int)n_elem, levels);
   if (top3d_last_error()) std::fprintf(stderr, "top3d: %s\n", top3d_last_error_string());
-  unsigned int loop = 0;
-  float change = 1;
-  DTYPE vol = 0.0;
-  int gciter_total = 0;
-  while ((change > 1e-2) && (loop < (unsigned)max_iterations)) {
-    loop++;
-    int cgiter = 0;
-    float cgres = 0;
-    solveStateMG_halo(&gpu_gc, xPhys, 4, (int)nl, cgtol, &solverData, &cgiter, &cgres, F, U);
-    vol = compute_volume(gridContext, xPhys);
-    getComplianceAndSensetivity_halo(&gpu_gc.target[0], xPhys, U, &c, dc);
-    applyDensityFilterGradient(gridContext, rmin, dc);
-    vol /= (DTYPE)(gridContext.nelx * gridContext.nely * gridContext.nelz);
-    const DTYPE g = vol - volfrac;
-    update_solution(x, xnew, dv, dc, (int)nelem, g);
-    change = compute_change(x, xnew, (int)nelem);
-    applyDensityFilter(gridContext, rmin, x, xPhys);
-    gciter_total += cgiter;
+  const DTYPE n_real = (DTYPE)(grid.nelx * grid.nely * grid.nelz);
+  unsigned int iteration = 0;
+  float max_change = 1;
+  DTYPE mean_density = 0.0, compliance = 0.0;
+  int cg_total = 0;
+  // solve, volume, sensitivity, filtered sensitivity, OC, change, filter
+  // (stencil_optimization.c:148-176)
+  while (max_change > 1e-2 && iteration < (unsigned)max_iterations) {
+    ++iteration;
+    int cg_last = 0;
+    float res_last = 0;
+    solveStateMG_halo(&devices, arr.phys, 4, levels, cgtol, &solver, &cg_last, &res_last, load,
+                      disp);
+    mean_density = compute_volume(grid, arr.phys) / n_real;
+    getComplianceAndSensetivity_halo(&devices.target[0], arr.phys, disp, &compliance,
+                                     arr.sens);
+    applyDensityFilterGradient(grid, rmin, arr.sens);
+    update_solution(arr.design, arr.next, arr.vol_sens, arr.sens, (int)n_elem,
+                    mean_density - volfrac);
+    max_change = compute_change(arr.design, arr.next, (int)n_elem);
+    applyDensityFilter(grid, rmin, arr.design, arr.phys);
+    cg_total += cg_last;
+    // the reference's per-iteration line (stencil_optimization.c:181-185)
     if (verbose == 1)
       std::printf("It.:%4i Obj.:%6.3e Vol.:%6.3f ch.:%4.2f relres: %4.2e iters: %4i time: %6.3f \n",
-                  loop, c, vol, change, cgres, cgiter, wtime());
+                  iteration, compliance, mean_density, max_change, res_last, cg_last, seconds());
   }
-  std::printf("%4i %12.6f %6.3f %4.2f %9i %9i %6.3f \n", loop, c, vol, change,
-              (int)(nelx * nely * nelz), gciter_total, wtime());
-  if (verbose == 1) std::printf("End time: %6.3f \n", wtime());
+  // the reference's summary line (stencil_optimization.c:193)
+  std::printf("%4i %12.6f %6.3f %4.2f %9i %9i %6.3f \n", iteration, compliance, mean_density,
+              max_change, (int)(nelx * nely * nelz), cg_total, seconds());
+  if (verbose == 1) std::printf("End time: %6.3f \n", seconds());
   if (write_result == 1) {
-    top3d_update_from(xPhys, sizeof(DTYPE) * nelem);
-    char name1[60], name2[60];
-    std::snprintf(name1, 60, "out_%d_%d_%d.vtu", (int)nelx, (int)nely, (int)nelz);
-    writeDensityVtkFile((int)nelx, (int)nely, (int)nelz, xPhys, name1);
-    std::snprintf(name2, 60, "out_%d_%d_%d_halo.vtu", (int)nelx, (int)nely, (int)nelz);
-    writeDensityVtkFileWithHalo((int)nelx, (int)nely, (int)nelz, xPhys, name2);
+    top3d_update_from(arr.phys, sizeof(DTYPE) * n_elem);
+    char path[64];
+    std::snprintf(path, sizeof(path), "out_%d_%d_%d.vtu", (int)nelx, (int)nely, (int)nelz);
+    writeDensityVtkFile((int)nelx, (int)nely, (int)nelz, arr.phys, path);
+    std::snprintf(path, sizeof(path), "out_%d_%d_%d_halo.vtu", (int)nelx, (int)nely, (int)nelz);
+    writeDensityVtkFileWithHalo((int)nelx, (int)nely, (int)nelz, arr.phys, path);
   }
-  for (void *p : {(void *)xPhys, (void *)x, (void *)dc, (void *)xnew, (void *)U, (void *)F,
-                  (void *)dv})
+  for (const void *p : {(const void *)arr.phys, (const void *)arr.design, (const void *)arr.sens,
+                        (const void *)arr.next, (const void *)disp, (const void *)load,
+                        (const void *)arr.vol_sens})
     top3d_unmap(p);
-  freeSolverData(&solverData, (int)nl);
-  freeGC(&gridContext, (int)nl);
-  freeGPUGrid(&gpu_gc, (int)nl);
-  std::free(F);
-  std::free(U);
-  std::free(dc);
-  std::free(dv);
-  std::free(x);
-  std::free(xPhys);
-  std::free(xnew);
+  freeSolverData(&solver, levels);
+  freeGC(&grid, levels);
+  freeGPUGrid(&devices, levels);
+  std::free(load);
+  std::free(disp);
+  arr.release();
 }
 
 }  // extern "C"

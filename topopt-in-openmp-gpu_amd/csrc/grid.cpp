@@ -234,6 +234,7 @@ void release_dev_grid(const struct gridContext *gc) {
   hipStreamSynchronize(stream());
   for (int l = 0; l < kMaxLevels; l++) {
     dfree(g->fixmask[l]);
+    dfree(g->fixplane[l]);
     dfree(g->fixed[l]);
   }
   for (int i = 0; i < 2; i++) {
@@ -259,10 +260,12 @@ void *DevGrid::work(int slot, size_t bytes) {
 // Upload fixed DOFs of level l (list + per-node mask) to the device.
 static bool upload_fixed(const struct gridContext *gc, DevGrid *g, int l) {
   dfree(g->fixmask[l]);
+  dfree(g->fixplane[l]);
   dfree(g->fixed[l]);
   const Level &L = g->lv[l];
   const int64_t n = (int64_t)gc->fixedDofs[l].n;
   std::vector<uint8_t> mask(L.nodes(), 0);
+  std::vector<int> plane(L.wx, 0);
   std::vector<int64_t> idx(n > 0 ? n : 1);
   for (int64_t f = 0; f < n; f++) {
     const int64_t d = (int64_t)gc->fixedDofs[l].idx[f];
@@ -272,12 +275,16 @@ static bool upload_fixed(const struct gridContext *gc, DevGrid *g, int l) {
     }
     idx[f] = d;
     mask[d / 3] |= (uint8_t)(1u << (d % 3));
+    plane[(d / 3) / ((int64_t)L.wy * L.wz)] = 1;
   }
   g->nfixed[l] = n;
   g->fixmask[l] = (uint8_t *)dalloc(mask.size());
+  g->fixplane[l] = (int *)dalloc(sizeof(int) * plane.size());
   g->fixed[l] = (int64_t *)dalloc(sizeof(int64_t) * idx.size());
-  if (!g->fixmask[l] || !g->fixed[l]) return false;
+  if (!g->fixmask[l] || !g->fixplane[l] || !g->fixed[l]) return false;
   return T3D_HIP(hipMemcpyAsync(g->fixmask[l], mask.data(), mask.size(),
+                                hipMemcpyHostToDevice, stream())) &&
+         T3D_HIP(hipMemcpyAsync(g->fixplane[l], plane.data(), sizeof(int) * plane.size(),
                                 hipMemcpyHostToDevice, stream())) &&
          T3D_HIP(hipMemcpyAsync(g->fixed[l], idx.data(), sizeof(int64_t) * n,
                                 hipMemcpyHostToDevice, stream())) &&

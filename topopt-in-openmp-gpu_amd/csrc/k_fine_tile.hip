@@ -29,6 +29,7 @@
 
 #include "t3d_internal.hpp"
 
+
 namespace t3d {
 
 // kModeJacobiDiag: Jacobi sweep whose inverse diagonal the kernel forms from
@@ -179,8 +180,8 @@ static constexpr uint32_t kK1Mask[24] = {
 constexpr int kK1Nnz = 279;
 // The non-zeros take 20 magnitudes: entry t of the row-major non-zero list is
 // kK1Class[t] & 127 of those, negated when bit 7 is set.  The classes hold
-// for every Poisson ratio (k1_classes.py derives them by sampling ratios from
-// 0.05 to 0.49); init_coef checks them against the K1 it computes.
+// for every Poisson ratio (tests/test_k1_classes.py derives them by sampling
+// ratios from 0.05 to 0.49); init_coef checks them against the K1 it computes.
 constexpr int kK1Classes = 20;
 static constexpr uint8_t kK1Class[kK1Nnz] = {
     0, 1, 129, 2, 130, 3, 130, 4, 3, 2, 132, 5, 1, 0, 1, 3,
@@ -336,14 +337,24 @@ struct TileArgs {
   int nty;               // tiles along y
 };
 
+// 1 / x: the hardware reciprocal (1 ulp) in FP32, where the inverse
+// diagonal only scales a smoother's correction; exact division in FP64.
+template <typename T>
+__device__ __forceinline__ T recip(T x) {
+  if constexpr (sizeof(T) == 4) return __builtin_amdgcn_rcpf(x);
+  else return T(1) / x;
+}
+
 // LVL 0: fine level, one modulus per element; LVL 1: level 1, eight fine
 // moduli per element.  MODE: apply / Jacobi sweep / residual.
+// fixplane[i] != 0 if a node of plane i has a fixed DOF; planes without one
+// skip the per-node mask.
 template <typename T, int LVL, int MODE, bool BCSYM>
 __device__ __forceinline__ void tile_sweep_body(
     TileArgs a, std::conditional_t<LVL == 0, Blocks<T>, K1Vals<T>> B, const T *__restrict__ E,
     const T *__restrict__ in, T *__restrict__ out, const T *__restrict__ b,
     const T *__restrict__ invD, T omega, const uint8_t *__restrict__ fix,
-    double *__restrict__ dotp) {
+    const int *__restrict__ fixplane, double *__restrict__ dotp) {
   static_assert(LVL == 0 || MODE != kModeJacobiDiag, "level 1 reads its inverse diagonal");
   constexpr bool kDiag = MODE == kModeJacobiDiag;
   constexpr int XS = kDiag ? 10 : 9;  // LDS values per thread and buffer
@@ -377,14 +388,14 @@ __device__ __forceinline__ void tile_sweep_body(
   const uint32_t n00c = elem_ok ? n00 : 0u;
   const uint32_t e00c = elem_ok ? (uint32_t)k * a.few + j : 0u;
   const uint32_t noff[2][2] = {{0u, (uint32_t)a.wy}, {1u, (uint32_t)a.wy + 1u}};  // [py][pz]
+  const uint32_t nown = (uint32_t)(k + 1) * a.wy + (uint32_t)(j + 1);  // owned node in a plane
   double dacc = 0.0;  // apply mode with dotp: partial sum of in . out
   // kDiag: moduli of the 4 elements of the previous slab at the owned node
   T es_prev = 0;
 
-  // transformed face of node plane i (zero outside the grid) and the raw
-  // values of its node (j+1, k+1)
-  auto load_face = [&](int i, T (&F)[4][3], T (&u11)[3]) {
-    T v[2][2][3];
+  // raw values of the face of node plane i (zero outside the grid)
+  auto fetch_face = [&](int i, T (&v)[2][2][3]) {
+    const bool pf = BCSYM && fixplane[i] != 0;
 #pragma unroll
     for (int py = 0; py < 2; py++)
 #pragma unroll
@@ -393,7 +404,7 @@ __device__ __forceinline__ void tile_sweep_body(
         const uint32_t nd = (uint32_t)i * sx + n00c + (elem_ok ? noff[py][pz] : 0u);
         const T *pn = in + 3u * nd;
         T v0 = pn[0], v1 = pn[1], v2 = pn[2];
-        if (BCSYM) {
+        if (BCSYM && pf) {
           const uint8_t m = fix[nd];
           if (m & 1) v0 = 0;
           if (m & 2) v1 = 0;
@@ -403,11 +414,8 @@ __device__ __forceinline__ void tile_sweep_body(
         v[py][pz][1] = v1;
         v[py][pz][2] = v2;
       }
-    u11[0] = v[1][1][0];
-    u11[1] = v[1][1][1];
-    u11[2] = v[1][1][2];
-    face_wht<T>(v, F);
   };
+  T vq[2][2][3];
 
   T pendT[4][3];  // transformed x+ face of the previous slab
 #pragma unroll
@@ -415,10 +423,40 @@ __device__ __forceinline__ void tile_sweep_body(
 #pragma unroll
     for (int c = 0; c < 3; c++) pendT[f][c] = 0;
 
-  // element slab i (node planes i, i+1): Fc/uc the transformed face and the
-  // owned raw values of plane i, Fn/un those of plane i+1 (loaded here)
-  auto step = [&](int i, const T (&Fc)[4][3], const T (&uc3)[3], T (&Fn)[4][3], T (&un3)[3]) {
-    load_face(i + 1, Fn, un3);
+  // element slab i (node planes i, i+1): Fc the transformed face of plane
+  // i, Fn that of plane i+1 (loaded here)
+  auto step = [&](int i, const T (&Fc)[4][3], T (&Fn)[4][3]) {
+    // the owned node's right-hand side, inverse diagonal and mask of plane
+    // i, loaded ahead of the element work that hides their latency
+    const bool emit = i >= i0;  // uniform
+    const uint32_t nd = (uint32_t)i * sx + nown;
+    const uint32_t d0 = 3u * nd;
+    // read only by owners (own_ok) of an emitted plane
+    T bb[3], di[3], u0[3];
+    uint8_t m;
+    if (own_ok) {  // (on the first, unemitted slab the loads are unused)
+      if (MODE != kModeApply) {
+        const T *pb = b + d0;
+        bb[0] = pb[0];
+        bb[1] = pb[1];
+        bb[2] = pb[2];
+      }
+      if (MODE == kModeJacobi) {
+        const T *pd = invD + d0;
+        di[0] = pd[0];
+        di[1] = pd[1];
+        di[2] = pd[2];
+      }
+      // the owned node's input: its value for the update and the
+      // identity rows (unmasked even when BCSYM masks the face loads)
+      const T *pn = in + d0;
+      u0[0] = pn[0];
+      u0[1] = pn[1];
+      u0[2] = pn[2];
+      m = fixplane[i] != 0 ? fix[nd] : uint8_t(0);
+    }
+    fetch_face(i + 1, vq);
+    face_wht<T>(vq, Fn);
     T h[8][3];
 #pragma unroll
     for (int f = 0; f < 4; f++)
@@ -427,6 +465,8 @@ __device__ __forceinline__ void tile_sweep_body(
         h[2 * f][c] = Fc[f][c] + Fn[f][c];
         h[2 * f + 1][c] = Fc[f][c] - Fn[f][c];
       }
+    // g[0][c] (the element's rigid translations) is zero at both levels and
+    // is never formed or read below
     T g[8][3];
     T Ee = 0;
     if constexpr (LVL == 0) {
@@ -479,12 +519,14 @@ __device__ __forceinline__ void tile_sweep_body(
       }
       element_core_l1<T>(B, Eh, h, g);
     }
-    if (i >= i0) {
+    if (emit) {
       // plane i: x- face of slab i plus the pending x+ face of slab i-1,
       // summed in the transformed basis
       T totT[4][3], tot[2][2][3];
 #pragma unroll
-      for (int f = 0; f < 4; f++)
+      for (int c = 0; c < 3; c++) totT[0][c] = g[1][c] + pendT[0][c];
+#pragma unroll
+      for (int f = 1; f < 4; f++)
 #pragma unroll
         for (int c = 0; c < 3; c++) totT[f][c] = (g[2 * f][c] + g[2 * f + 1][c]) + pendT[f][c];
       face_iwht<T>(totT, tot);
@@ -517,48 +559,25 @@ __device__ __forceinline__ void tile_sweep_body(
           for (int c = 0; c < 3; c++) kin[c] += buf[XS * (t + a.TY + 1) + 6 + c];
           if constexpr (kDiag) es += buf[XS * (t + a.TY + 1) + 9];
         }
-        const uint32_t nd = (uint32_t)i * sx + (uint32_t)(k + 1) * a.wy + (uint32_t)(j + 1);
-        const uint32_t d0 = 3u * nd;
-        const T *pin = in + d0;
-        const uint8_t m = fix[nd];
-        T bb[3] = {0, 0, 0}, dd[3] = {0, 0, 0};
-        if (MODE != kModeApply) {
-          const T *pb = b + d0;
-          bb[0] = pb[0];
-          bb[1] = pb[1];
-          bb[2] = pb[2];
-        }
+        T dd[3] = {di[0], di[1], di[2]};
         if constexpr (kDiag) {
           // diagonal KE[c][c] (es_prev + es); identity rows at fixed DOFs
-          const T rs = T(1) / (es_prev + es);
+          const T rs = recip<T>(es_prev + es);
           es_prev = es;
 #pragma unroll
           for (int c = 0; c < 3; c++) dd[c] = ((m >> c) & 1) ? T(1) : rs * B.ikd[c];
-        } else if (MODE == kModeJacobi) {
-          const T *pd = invD + d0;
-          dd[0] = pd[0];
-          dd[1] = pd[1];
-          dd[2] = pd[2];
         }
         T o[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-          const T uc = uc3[c];
+          const T uc = u0[c];
           T kv = kin[c];
-          if (BCSYM) {  // the loaded value is masked; the BC row needs in
-            if ((m >> c) & 1) kv = pin[c];
-          } else {
-            if ((m >> c) & 1) kv = uc;
-          }
+          if ((m >> c) & 1) kv = uc;  // identity row
           if (MODE == kModeApply) {
             o[c] = kv;
-            if (dotp) {
-              const T pv = (BCSYM && ((m >> c) & 1)) ? pin[c] : uc;
-              dacc += (double)pv * (double)o[c];
-            }
+            if (dotp) dacc += (double)uc * (double)o[c];
           } else if (MODE == kModeJacobi || kDiag) {
-            const T uo = BCSYM ? pin[c] : uc;
-            o[c] = uo + omega * dd[c] * (bb[c] - kv);
+            o[c] = uc + omega * dd[c] * (bb[c] - kv);
           } else {
             o[c] = bb[c] - kv;
           }
@@ -570,21 +589,23 @@ __device__ __forceinline__ void tile_sweep_body(
       }
     }
 #pragma unroll
-    for (int f = 0; f < 4; f++)
+    for (int c = 0; c < 3; c++) pendT[0][c] = -g[1][c];
+#pragma unroll
+    for (int f = 1; f < 4; f++)
 #pragma unroll
       for (int c = 0; c < 3; c++) pendT[f][c] = g[2 * f][c] - g[2 * f + 1][c];
   };
 
-  T Fa[4][3], Fb[4][3], ua[3], ub[3];
-  load_face(i0 - 1, Fa, ua);
+  // the march; the transformed face of plane i+1 becomes that of plane i
+  T Fa[4][3], Fb[4][3];
+  fetch_face(i0 - 1, vq);
+  face_wht<T>(vq, Fa);
   for (int i = i0 - 1; i <= i1; i++) {
-    step(i, Fa, ua, Fb, ub);
+    step(i, Fa, Fb);
 #pragma unroll
     for (int f = 0; f < 4; f++)
 #pragma unroll
       for (int c = 0; c < 3; c++) Fa[f][c] = Fb[f][c];
-#pragma unroll
-    for (int c = 0; c < 3; c++) ua[c] = ub[c];
   }
   if (MODE == kModeApply && dotp) {  // one partial per workgroup, fixed order
 #pragma unroll
@@ -606,18 +627,19 @@ __global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_tile_sweep(
     TileArgs a, std::conditional_t<LVL == 0, Blocks<T>, K1Vals<T>> B, const T *__restrict__ E,
     const T *__restrict__ in, T *__restrict__ out, const T *__restrict__ b,
     const T *__restrict__ invD, T omega, const uint8_t *__restrict__ fix,
-    double *__restrict__ dotp) {
-  tile_sweep_body<T, LVL, MODE, BCSYM>(a, B, E, in, out, b, invD, omega, fix, dotp);
+    const int *__restrict__ fixplane, double *__restrict__ dotp) {
+  tile_sweep_body<T, LVL, MODE, BCSYM>(a, B, E, in, out, b, invD, omega, fix, fixplane, dotp);
 }
 
-// fine level in FP32: workgroups of up to 1024 threads; 74 VGPRs, so two
-// 12-wave workgroups of 65 x 11 columns share a CU
+// fine level in FP32: workgroups of up to 1024 threads; two 9-wave
+// workgroups of 33 x 17 columns share a CU
 template <int MODE, bool BCSYM>
 __global__ __launch_bounds__(1024) void k_tile_sweep_f32(
     TileArgs a, Blocks<float> B, const float *__restrict__ E, const float *__restrict__ in,
     float *__restrict__ out, const float *__restrict__ b, const float *__restrict__ invD,
-    float omega, const uint8_t *__restrict__ fix, double *__restrict__ dotp) {
-  tile_sweep_body<float, 0, MODE, BCSYM>(a, B, E, in, out, b, invD, omega, fix, dotp);
+    float omega, const uint8_t *__restrict__ fix, const int *__restrict__ fixplane,
+    double *__restrict__ dotp) {
+  tile_sweep_body<float, 0, MODE, BCSYM>(a, B, E, in, out, b, invD, omega, fix, fixplane, dotp);
 }
 
 // ---------------------------------------------------------------------------
@@ -699,11 +721,14 @@ struct TileShape {
 // override: TOP3D_TILE_L<level>=TY,TZ,NX (NX = 0: from the occupancy).
 //
 // Shapes measured on MI355X at 256x128x128 (fine) / 128x64x64 (level 1):
-// for the FP32 fine level 65 x 11 (715 threads in 12 waves, two workgroups
-// per CU; z tiles of 11 element rows own 10) ran 44.7 us in the V-cycle
-// against 47.3 us for 65 x 7, 46.1 us for 65 x 9, 48.6 us for 33 x 22 and
-// 55.6 us for 65 x 15 (one workgroup per CU); wide, short x-marches for
-// level 1:
+// for the FP32 fine level 33 x 17 (561 threads in 9 waves, two workgroups
+// per CU, 32 tiles of 32 x 16 owned node columns) ran 42.8 us in the
+// V-cycle against 44.9 us for 65 x 11 (12 waves), 44.0 us for 44 x 17,
+// 47.3 us for 65 x 7, 47.7 us for 33 x 23 and 55.3 us for 65 x 12 (one
+// workgroup per CU); for the FP64 fine level 17 x 12 (4 waves) ran the
+// outer CG's apply in 73.6 us (repeated on one buffer) against 75.8 us
+// for 65 x 7, 81-112 us for 33 x 7/12/15, 25 x 20, 45 x 11 and 17 x 24;
+// wide, short x-marches for level 1:
 // in FP32, 65 x 11 (12 waves, 3 per SIMD, 231 workgroups) ran 17.6 us against
 // 18.6 us for 65 x 12 (13 waves, 198 workgroups).  The x-march length
 // follows from the number of workgroups the device holds at once (`slots`):
@@ -720,13 +745,14 @@ static TileShape tile_shape(const Level &L, int lvl, int elem_bytes, int &nx_fix
   struct Rule {
     int ycap, tz, nx;
   };
-  static constexpr Rule kRules[2][2] = {{{17, 12, 0}, {65, 11, 0}},  // level 0: f64, f32
+  static constexpr Rule kRules[2][2] = {{{17, 12, 0}, {33, 17, 0}},  // level 0: f64, f32
                                         {{65, 6, 8}, {65, 11, 4}}};  // level 1
   const Rule r = kRules[lvl == 0 ? 0 : 1][elem_bytes == 4 ? 1 : 0];
   s.TY = std::max(2, std::min(L.nely + 1, r.ycap));
   s.TZ = std::max(2, std::min({L.nelz + 1, r.tz, max_threads / s.TY}));
   nx_fixed = r.nx;
-  const char *env = std::getenv(lvl == 0 ? "TOP3D_TILE_L0" : "TOP3D_TILE_L1");
+  const char *env = std::getenv(lvl == 0 ? (elem_bytes == 8 ? "TOP3D_TILE_L0_F64" : "TOP3D_TILE_L0")
+                                          : "TOP3D_TILE_L1");
   int oy = 0, oz = 0, ox = 0;
   if (env && std::sscanf(env, "%d,%d,%d", &oy, &oz, &ox) == 3 && oy >= 2 && oz >= 2 && ox >= 0 &&
       oy * oz <= max_threads) {
@@ -856,10 +882,10 @@ static int launch_tile(const DevGrid &g, int mode, const T *E, const T *in, T *o
   do {                                                                                       \
     if constexpr (kF32Fine)                                                                  \
       hipLaunchKernelGGL((k_tile_sweep_f32<M, S>), grid, dim3(bs), shm, stream(), a, B, E,   \
-                         in, out, b, invD, omega, g.fixmask[LVL], dotp);                     \
+                         in, out, b, invD, omega, g.fixmask[LVL], g.fixplane[LVL], dotp);    \
     else                                                                                     \
       hipLaunchKernelGGL((k_tile_sweep<T, LVL, M, S>), grid, dim3(bs), shm, stream(), a, B, \
-                         E, in, out, b, invD, omega, g.fixmask[LVL], dotp);                  \
+                         E, in, out, b, invD, omega, g.fixmask[LVL], g.fixplane[LVL], dotp); \
   } while (0)
   if (bcsym) {
     if (mode == kModeApply) T3D_LAUNCH(kModeApply, true);

@@ -66,6 +66,7 @@ struct DevGrid {
   int64_t nelem = 0;      // padded fine element count (wrapx-1)(wrapy-1)(wrapz-1)
   double E0 = 1.0, Emin = 1e-6;
   uint8_t *fixmask[kMaxLevels] = {};   // per halo node: bit c set = DOF fixed
+  int *fixplane[kMaxLevels] = {};      // per node plane i: 1 if a node of it has a fixed DOF
   int64_t nfixed[kMaxLevels] = {};
   int64_t *fixed[kMaxLevels] = {};     // fixed DOF indices (device)
   double *KE[2] = {};                  // device KE_0 (576), KE_1 (8*576)
