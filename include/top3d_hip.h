@@ -305,6 +305,26 @@ void top3d_line_load_profile(const struct gridContext *gc, CTYPE *F, int profile
 void top3d_set_default_support(int mode);
 int top3d_get_default_support(void);
 
+/* Layout adapter for arrays in the reference's padding.  The reference pads
+ * y so that nely_l + 1 + pad is a multiple of STENCIL_SIZE_Y = 128
+ * (computePadding, gpu_definitions.c:67-92); this library's computePadding
+ * does not pad y.  Node (i, j, k) and element (i, j, k) keep their halo
+ * indices in both, so these copy each (i, k) row; the destination's
+ * padding is zero-filled.  Host arrays; node arrays have 3 DOFs per node,
+ * element arrays are level 0.  top3d_reference_padding gives the
+ * reference's wrap sizes and ndof for level l.  No counterpart in the
+ * reference.                                                              */
+void top3d_reference_padding(const struct gridContext *gc, int l, int *wrapx,
+                             int *wrapy, int *wrapz, uint64_t *ndof);
+void top3d_nodes_from_reference_layout(const struct gridContext *gc, int l,
+                                       const double *ref, double *lib);
+void top3d_nodes_to_reference_layout(const struct gridContext *gc, int l,
+                                     const double *lib, double *ref);
+void top3d_elements_from_reference_layout(const struct gridContext *gc,
+                                          const double *ref, double *lib);
+void top3d_elements_to_reference_layout(const struct gridContext *gc,
+                                        const double *lib, double *ref);
+
 /* present table (replaces `#pragma omp target enter/exit data/update`) */
 int top3d_map(const void *host, uint64_t bytes);          /* alloc + copy in */
 int top3d_map_alloc(const void *host, uint64_t bytes);    /* alloc, no copy */
@@ -368,6 +388,8 @@ int top3d_compliance_backward(const struct gpuGrid *gpu_gc, double rmin,
  *   2 fine operator, FP64 (the outer CG matvec q = K p)
  *   3 level-1 damped-Jacobi sweep, FP32
  *   4 level-1 damped-Jacobi sweep, FP64
+ *   5 fine operator, FP64, with the per-workgroup partial sums of p.q (the
+ *     form the CG iteration launches)
  * Outputs the mean milliseconds per launch and the algorithmic HBM bytes of
  * one launch (true nodes and elements only: vectors read/written once,
  * moduli read once).                                                         */

@@ -203,3 +203,53 @@ def test_filter_radii(pair, rmin):
     v = g.density(rng.uniform(-1, 1, inter.size))
     assert relerr(g.filter_grad(rmin, v).cpu().numpy()[inter],
                   o.filter_grad(rmin, v.cpu().numpy())[inter]) <= 1e-14
+
+
+def test_reference_layout_adapter_operator_and_solve():
+    """Arrays in the reference's padded layout (y to a multiple of 128,
+    gpu_definitions.c:67-92) go through the adapter into the library and
+    back: the fine operator matches the oracle built with that padding to
+    1e-13, and a solve on its load and densities gives its displacement."""
+    nelx, nely, nelz, nl = 16, 8, 8, 3
+    g = top3d_amd.Grid(nelx, nely, nelz, nl)
+    o = pyoracle.Grid(nelx, nely, nelz, nl, yalign=128)
+    lib, gc = g.lib, ctypes.byref(g.gc)
+    P = ctypes.c_void_p
+    rng = np.random.default_rng(9)
+    _, wx, wy, wz, ndof_ref = o.padding(0)
+    i, k, j = np.meshgrid(np.arange(1, nelx + 1), np.arange(1, nelz + 1),
+                          np.arange(1, nely + 1), indexing="ij")
+    x_ref = np.zeros(o.nelem())
+    x_ref[(i * (wy - 1) * (wz - 1) + k * (wy - 1) + j).ravel()] = rng.uniform(
+        0.1, 1.0, nelx * nely * nelz)
+    v_ref = np.zeros(ndof_ref)
+    node = np.zeros((wx, wz, wy), bool)
+    node[1:nelx + 2, 1:nelz + 2, 1:nely + 2] = True
+    v_ref[np.repeat(node.ravel(), 3)] = rng.standard_normal(3 * node.sum())
+    x_lib = np.zeros(g.nelem)
+    v_lib = np.zeros(g.ndof(0))
+    lib.top3d_elements_from_reference_layout(gc, P(x_ref.ctypes.data), P(x_lib.ctypes.data))
+    lib.top3d_nodes_from_reference_layout(gc, 0, P(v_ref.ctypes.data), P(v_lib.ctypes.data))
+    out_lib = g.apply(cuda(x_lib), cuda(v_lib)).cpu().numpy()
+    out_ref = np.zeros(ndof_ref)
+    lib.top3d_nodes_to_reference_layout(gc, 0, P(out_lib.ctypes.data), P(out_ref.ctypes.data))
+    want = o.matvec0(x_ref, v_ref)
+    real = np.repeat(node.ravel(), 3)
+    assert relerr(out_ref[real], want[real]) <= 1e-13
+    # the solve through the adapter: the oracle's load and uniform densities
+    F_ref = o.load()
+    xu_ref = np.where(x_ref > 0, float(np.float32(0.2)), 0.0)
+    F_lib, xu_lib = np.zeros(g.ndof(0)), np.zeros(g.nelem)
+    lib.top3d_nodes_from_reference_layout(gc, 0, P(F_ref.ctypes.data), P(F_lib.ctypes.data))
+    lib.top3d_elements_from_reference_layout(gc, P(xu_ref.ctypes.data), P(xu_lib.ctypes.data))
+    U = g.zeros(0)
+    it, _ = g.solve(cuda(xu_lib), cuda(F_lib), U)
+    U_ref = np.zeros(ndof_ref)
+    U_host = U.cpu().numpy()
+    lib.top3d_nodes_to_reference_layout(gc, 0, P(U_host.ctypes.data), P(U_ref.ctypes.data))
+    Uo = np.zeros(ndof_ref)
+    ito, _ = o.solve(xu_ref, F_ref, Uo)
+    assert it == ito
+    assert np.linalg.norm(U_ref - Uo) <= 1e-9 * np.linalg.norm(Uo)
+    g.close()
+    o.close()

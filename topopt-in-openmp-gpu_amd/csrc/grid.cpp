@@ -1,5 +1,6 @@
 // Library context (error state, stream, present table), grid set-up and the
 // device-side grid registry.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdlib>
@@ -483,6 +484,84 @@ void computePadding(const struct gridContext *gc, const int l, int *ncell,
   *wrapy = L.wy;
   *wrapz = L.wz;
   *ndof = (uint_fast32_t)L.ndof;
+}
+
+// ---- the reference's padded layout ------------------------------------------
+// The reference pads y so that nely_l + 1 + pad is a multiple of
+// STENCIL_SIZE_Y = 128 (gpu_definitions.c:67-92); x and z carry no padding
+// (STENCIL_SIZE_X = STENCIL_SIZE_Z = 1, definitions.h:20-21).  This library
+// leaves y unpadded.  Both keep node (i, j, k) at i*wy*wz + k*wy + j with the
+// same halo, so a copy moves each (i, k) row of y-length min(wy) as a block.
+namespace {
+struct Strides {
+  int64_t wx, wy_ref, wy_lib, wz;
+};
+Strides layout_strides(const struct gridContext *gc, int l) {
+  Level L;
+  compute_level(gc, l, &L);
+  const int ny = L.nely;
+  const int pad = (STENCIL_SIZE_Y - ((ny + 1) % STENCIL_SIZE_Y)) % STENCIL_SIZE_Y;
+  return Strides{L.wx, (int64_t)ny + pad + 3, L.wy, L.wz};
+}
+// rows of `per` values each: (rows_x * rows_z) rows, `len` values copied,
+// the destination's tail of each row zeroed
+void copy_rows(int64_t nrows, int64_t src_row, int64_t dst_row, int64_t per, const double *src,
+               double *dst) {
+  const int64_t len = std::min(src_row, dst_row) * per;
+  for (int64_t r = 0; r < nrows; r++) {
+    std::memcpy(dst + r * dst_row * per, src + r * src_row * per, sizeof(double) * len);
+    std::fill(dst + r * dst_row * per + len, dst + (r + 1) * dst_row * per, 0.0);
+  }
+}
+bool layout_args_ok(const struct gridContext *gc, int l, const void *a, const void *b,
+                    const char *what) {
+  if (!gc || !a || !b || l < 0 || l >= kMaxLevels || (gc->nely >> l) < 1) {
+    set_error(TOP3D_ERR_ARG, "%s: invalid arguments", what);
+    return false;
+  }
+  return true;
+}
+}  // namespace
+
+void top3d_nodes_from_reference_layout(const struct gridContext *gc, int l, const double *ref,
+                                       double *lib) {
+  if (!layout_args_ok(gc, l, ref, lib, "top3d_nodes_from_reference_layout")) return;
+  const Strides s = layout_strides(gc, l);
+  copy_rows(s.wx * s.wz, s.wy_ref, s.wy_lib, 3, ref, lib);
+}
+
+void top3d_nodes_to_reference_layout(const struct gridContext *gc, int l, const double *lib,
+                                     double *ref) {
+  if (!layout_args_ok(gc, l, lib, ref, "top3d_nodes_to_reference_layout")) return;
+  const Strides s = layout_strides(gc, l);
+  copy_rows(s.wx * s.wz, s.wy_lib, s.wy_ref, 3, lib, ref);
+}
+
+void top3d_elements_from_reference_layout(const struct gridContext *gc, const double *ref,
+                                          double *lib) {
+  if (!layout_args_ok(gc, 0, ref, lib, "top3d_elements_from_reference_layout")) return;
+  const Strides s = layout_strides(gc, 0);
+  copy_rows((s.wx - 1) * (s.wz - 1), s.wy_ref - 1, s.wy_lib - 1, 1, ref, lib);
+}
+
+void top3d_elements_to_reference_layout(const struct gridContext *gc, const double *lib,
+                                        double *ref) {
+  if (!layout_args_ok(gc, 0, lib, ref, "top3d_elements_to_reference_layout")) return;
+  const Strides s = layout_strides(gc, 0);
+  copy_rows((s.wx - 1) * (s.wz - 1), s.wy_lib - 1, s.wy_ref - 1, 1, lib, ref);
+}
+
+void top3d_reference_padding(const struct gridContext *gc, int l, int *wrapx, int *wrapy,
+                             int *wrapz, uint64_t *ndof) {
+  if (!gc || l < 0 || l >= kMaxLevels) {
+    set_error(TOP3D_ERR_ARG, "top3d_reference_padding: invalid arguments");
+    return;
+  }
+  const Strides s = layout_strides(gc, l);
+  if (wrapx) *wrapx = (int)s.wx;
+  if (wrapy) *wrapy = (int)s.wy_ref;
+  if (wrapz) *wrapz = (int)s.wz;
+  if (ndof) *ndof = 3 * (uint64_t)s.wx * s.wy_ref * s.wz;
 }
 
 // stencil_grid_utility.c:7-65.  TOP3D_SUPPORT_STRIPS (the default) clamps

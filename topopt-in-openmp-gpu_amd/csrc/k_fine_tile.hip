@@ -357,7 +357,9 @@ __device__ __forceinline__ void tile_sweep_body(
     const int *__restrict__ fixplane, double *__restrict__ dotp) {
   static_assert(LVL == 0 || MODE != kModeJacobiDiag, "level 1 reads its inverse diagonal");
   constexpr bool kDiag = MODE == kModeJacobiDiag;
-  constexpr int XS = kDiag ? 10 : 9;  // LDS values per thread and buffer
+  // LDS values per thread and buffer: 9, or 10 with the modulus, padded to
+  // an odd stride so that 64 consecutive threads hit distinct banks
+  constexpr int XS = kDiag ? 11 : 9;
   extern __shared__ unsigned char smem_raw[];
   T *xch = reinterpret_cast<T *>(smem_raw);  // [2][nthreads][XS]
   const int nthr = a.TY * a.TZ;
@@ -737,9 +739,9 @@ struct TileShape {
 // rounds x (NX + 3), the per-workgroup cost in slab steps.
 static TileShape tile_shape(const Level &L, int lvl, int elem_bytes, int &nx_fixed) {
   TileShape s{};
-  // LDS values per thread: 2 buffers of 9 (10 on the fine level, whose
-  // diagonal-forming sweep also exchanges a modulus)
-  const int lds_threads = 80 * 1024 / ((lvl == 0 ? 20 : 18) * elem_bytes);
+  // LDS values per thread: 2 buffers of 9 (11 on the fine level, whose
+  // diagonal-forming sweep also exchanges a modulus; padded to an odd stride)
+  const int lds_threads = 80 * 1024 / ((lvl == 0 ? 22 : 18) * elem_bytes);
   // launch bounds: 1024 threads in FP32, 512 in FP64
   const int max_threads = std::min(elem_bytes == 4 ? 1024 : 512, lds_threads);
   struct Rule {
@@ -839,7 +841,7 @@ static int launch_tile(const DevGrid &g, int mode, const T *E, const T *in, T *o
   TileShape s = tile_shape(L, LVL, (int)sizeof(T), nx_fixed);
   const int nthr = s.TY * s.TZ;
   const int bs = (nthr + 63) / 64 * 64;
-  const size_t shm = sizeof(T) * 2 * (diag ? 10 : 9) * (size_t)nthr;
+  const size_t shm = sizeof(T) * 2 * (diag ? 11 : 9) * (size_t)nthr;
   // per instantiation, [0] sweeps reading invD, [1] forming the diagonal
   static int cached_nthr[2] = {-1, -1}, cached_slots[2] = {1, 1};
   constexpr bool kF32Fine = sizeof(T) == 4 && LVL == 0;

@@ -505,7 +505,7 @@ int top3d_time_kernel(struct SolverDataBuffer *data, int which, int reps, double
                       double *bytes) {
   SolverState *s = solver_state(data);
   if (!s) return top3d_last_error();
-  if (reps < 1 || which < 0 || which > 4) {
+  if (reps < 1 || which < 0 || which > 5) {
     set_error(TOP3D_ERR_ARG, "top3d_time_kernel: which=%d reps=%d", which, reps);
     return TOP3D_ERR_ARG;
   }
@@ -537,6 +537,10 @@ int top3d_time_kernel(struct SolverDataBuffer *data, int which, int reps, double
       }
       case 2:
         launch_fine<double>(g, 0, s->E64, s->p, s->q, nullptr, nullptr, 0.0, s->bcsym);
+        break;
+      case 5:  // as the CG runs it: with the per-workgroup partial sums of p.q
+        launch_fine_apply_dot(g, s->E64, s->p, s->q, s->bcsym, s->cgpart + SolverState::kPartRZ,
+                              SolverState::kPartPQ);
         break;
       case 3: {
         float *a = s->v32[1][SolverState::kZ], *b = s->t32[1];
@@ -570,7 +574,8 @@ int top3d_time_kernel(struct SolverDataBuffer *data, int which, int reps, double
     switch (which) {
       case 0: *bytes = 12.0 * vec * nn0 + 4.0 * ne0; break;
       case 1: *bytes = 24.0 * vec * nn0 + 8.0 * ne0; break;
-      case 2: *bytes = 48.0 * nn0 + 8.0 * ne0; break;   // p in, q out, E
+      case 2:
+      case 5: *bytes = 48.0 * nn0 + 8.0 * ne0; break;   // p in, q out, E
       case 3: *bytes = 48.0 * nn1 + 4.0 * ne0; break;   // level-1 vectors, fine E
       default: *bytes = 96.0 * nn1 + 8.0 * ne0;
     }

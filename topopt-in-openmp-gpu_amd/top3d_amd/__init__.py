@@ -169,6 +169,11 @@ SIGNATURES = {
     "top3d_dense_spd_inverse": (c_int, [P, c_int, c_int]),
     "top3d_time_kernel": (c_int, [P, c_int, c_int, P, P]),
     "top3d_memory_info": (c_int, [P, c_int]),
+    "top3d_reference_padding": (None, [P, c_int, P, P, P, P]),
+    "top3d_nodes_from_reference_layout": (None, [P, c_int, P, P]),
+    "top3d_nodes_to_reference_layout": (None, [P, c_int, P, P]),
+    "top3d_elements_from_reference_layout": (None, [P, P, P]),
+    "top3d_elements_to_reference_layout": (None, [P, P, P]),
     "top3d_sweep_timing": (c_int, [P, c_int, P, P, P]),
     "applyStateOperator_stencil_f32": (None, [P, P, P, P]),
     "applyStateOperatorSubspace_halo_f32": (None, [P, c_int, P, P, P]),
@@ -448,7 +453,7 @@ class Grid:
                     solve_ms=s.solve_ms, status=s.status)
 
     KERNELS = {"fine_sweep_f32": 0, "fine_sweep_f64": 1, "fine_apply_f64": 2,
-               "level1_sweep_f32": 3, "level1_sweep_f64": 4}
+               "level1_sweep_f32": 3, "level1_sweep_f64": 4, "fine_apply_dot_f64": 5}
 
     def time_kernel(self, name, reps=20):
         """(mean ms per launch, algorithmic bytes per launch) of a hot kernel."""
