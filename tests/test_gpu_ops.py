@@ -253,3 +253,51 @@ def test_reference_layout_adapter_operator_and_solve():
     assert np.linalg.norm(U_ref - Uo) <= 1e-9 * np.linalg.norm(Uo)
     g.close()
     o.close()
+
+
+def test_coarse_factor_at_production_size():
+    """The coarsest level of every benchmark configuration (coarse 16x8x8,
+    n = 4131 DOFs, nested-dissection tiled Cholesky and W = L^-1 as the
+    solver uses them) through initializeCholmod / factorizeSubspaceMatrix /
+    solveSubspaceMatrix (stencil_assembly.c:366-503), against a numpy dense
+    solve of the same CSR matrix, on random densities in [0.1, 1]."""
+    nelx, nely, nelz, nl = 64, 32, 32, 3
+    print("[gpu coarse] 64x32x32 nl=3: coarse n = 4131", flush=True)
+    g = top3d_amd.Grid(nelx, nely, nelz, nl)
+    lib = g.lib
+    rng = np.random.default_rng(21)
+    xg = g.density(rng.uniform(0.1, 1.0, nelx * nely * nelz))
+    l = nl - 1
+    M = top3d_amd.CSRMatrix()
+    lib.allocateSubspaceMatrix(g.gc, l, ctypes.byref(M))
+    lib.assembleSubspaceMatrix(g.gc, l, top3d_amd._ptr(xg), M, None)
+    top3d_amd.check("assembleSubspaceMatrix")
+    n = M.nrows
+    assert n == 4131
+    ro = np.ctypeslib.as_array(M.rowOffsets, (n + 1,)).copy()
+    ci = np.ctypeslib.as_array(M.colIndex, (M.nnz,)).copy()
+    va = np.ctypeslib.as_array(M.vals, (M.nnz,)).copy()
+    A = np.zeros((n, n))
+    for r in range(n):
+        A[r, ci[ro[r]:ro[r + 1]]] = va[ro[r]:ro[r + 1]]
+    cs = top3d_amd.CoarseSolverData()
+    lib.initializeCholmod(g.gc, l, ctypes.byref(cs), M)
+    lib.factorizeSubspaceMatrix(g.gc, l, cs, M)
+    top3d_amd.check("factorizeSubspaceMatrix")
+    lay = fem_ref.Layout(nelx, nely, nelz, l)
+    inter = lay.interior_dofs()
+    assert inter.size == n
+    for trial in range(2):
+        rhs = np.zeros(g.ndof(l))
+        rhs[inter] = rng.standard_normal(n)
+        out = g.zeros(l)
+        lib.solveSubspaceMatrix(g.gc, l, cs, top3d_amd._ptr(cuda(rhs)), top3d_amd._ptr(out))
+        top3d_amd.check("solveSubspaceMatrix")
+        want = np.linalg.solve(A, rhs[inter])
+        got = out.cpu().numpy()[inter]
+        err = np.linalg.norm(got - want) / np.linalg.norm(want)
+        print(f"[gpu coarse] trial {trial}: rel err {err:.2e}", flush=True)
+        assert err <= 1e-10
+    lib.finishCholmod(g.gc, l, ctypes.byref(cs), M, 0)
+    lib.freeSubspaceMatrix(ctypes.byref(M))
+    g.close()
