@@ -337,6 +337,19 @@ struct TileArgs {
   int nty;               // tiles along y
 };
 
+// p[idx] with the byte offset formed in 32 bits, so that the load or store
+// takes the uniform base in scalar registers and a 32-bit offset (no 64-bit
+// address arithmetic per access); the host checks that every byte offset of
+// the level's arrays fits in 32 bits.
+template <typename T>
+__device__ __forceinline__ const T *at32(const T *p, uint32_t idx) {
+  return reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + idx * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ T *at32(T *p, uint32_t idx) {
+  return reinterpret_cast<T *>(reinterpret_cast<char *>(p) + idx * (uint32_t)sizeof(T));
+}
+
 // 1 / x: the hardware reciprocal (1 ulp) in FP32, where the inverse
 // diagonal only scales a smoother's correction; exact division in FP64.
 template <typename T>
@@ -380,7 +393,7 @@ __device__ __forceinline__ void tile_sweep_body(
                       (!last_z || k + 1 == a.nelz + 1);
   // level 1: the element is real (its fine moduli exist)
   const bool real_yz = elem_ok && j >= 1 && k >= 1;
-  // 32-bit offsets: the host checks 3 * nodes < 2^31
+  // 32-bit offsets: the host checks sizeof(T) * 3 * nodes < 2^32
   const uint32_t sx = (uint32_t)a.wy * (uint32_t)a.wz;
   const uint32_t n00 = (uint32_t)k * a.wy + j;  // node (j, k) within a plane
   // Threads whose element column lies outside the grid read the halo node
@@ -404,7 +417,7 @@ __device__ __forceinline__ void tile_sweep_body(
       for (int pz = 0; pz < 2; pz++) {
         // (the march only visits planes i <= nelx + 2)
         const uint32_t nd = (uint32_t)i * sx + n00c + (elem_ok ? noff[py][pz] : 0u);
-        const T *pn = in + 3u * nd;
+        const T *pn = at32(in, 3u * nd);
         T v0 = pn[0], v1 = pn[1], v2 = pn[2];
         if (BCSYM && pf) {
           const uint8_t m = fix[nd];
@@ -438,20 +451,20 @@ __device__ __forceinline__ void tile_sweep_body(
     uint8_t m;
     if (own_ok) {  // (on the first, unemitted slab the loads are unused)
       if (MODE != kModeApply) {
-        const T *pb = b + d0;
+        const T *pb = at32(b, d0);
         bb[0] = pb[0];
         bb[1] = pb[1];
         bb[2] = pb[2];
       }
       if (MODE == kModeJacobi) {
-        const T *pd = invD + d0;
+        const T *pd = at32(invD, d0);
         di[0] = pd[0];
         di[1] = pd[1];
         di[2] = pd[2];
       }
       // the owned node's input: its value for the update and the
       // identity rows (unmasked even when BCSYM masks the face loads)
-      const T *pn = in + d0;
+      const T *pn = at32(in, d0);
       u0[0] = pn[0];
       u0[1] = pn[1];
       u0[2] = pn[2];
@@ -473,15 +486,15 @@ __device__ __forceinline__ void tile_sweep_body(
     T Ee = 0;
     if constexpr (LVL == 0) {
       // (the march only visits slabs i <= nelx + 1)
-      Ee = E[(uint32_t)i * a.fez * a.few + e00c];
+      Ee = *at32(E, (uint32_t)i * a.fez * a.few + e00c);
       element_core<T>(B, Ee, h, g);
       if constexpr (kDiag) {
         if (i < i0 && own_ok) {  // first slab: the neighbours' moduli from memory
           const uint32_t eb = ((uint32_t)i * a.fez + k) * a.few + j;
           T es = Ee;
-          if (!last_y) es += E[eb + 1];
-          if (!last_z) es += E[eb + a.few];
-          if (!last_y && !last_z) es += E[eb + a.few + 1];
+          if (!last_y) es += *at32(E, eb + 1);
+          if (!last_z) es += *at32(E, eb + a.few);
+          if (!last_y && !last_z) es += *at32(E, eb + a.few + 1);
           es_prev = es;
         }
       }
@@ -494,7 +507,7 @@ __device__ __forceinline__ void tile_sweep_body(
         for (int o = 0; o < 8; o++) {
           const uint32_t fe = ((uint32_t)(2 * i - 1 + (o & 1)) * a.fez + (2 * k - 1 + (o >> 2))) *
                                   a.few + (2 * j - 1 + ((o >> 1) & 1));
-          Eo[o] = E[fe];
+          Eo[o] = *at32(E, fe);
         }
       }
       // Ehat = per-element 8-point transform of the moduli
@@ -584,7 +597,7 @@ __device__ __forceinline__ void tile_sweep_body(
             o[c] = bb[c] - kv;
           }
         }
-        T *po = out + d0;
+        T *po = at32(out, d0);
         po[0] = o[0];
         po[1] = o[1];
         po[2] = o[2];
@@ -828,7 +841,8 @@ static int launch_tile(const DevGrid &g, int mode, const T *E, const T *in, T *o
     return 0;
   }
   const Level &L = g.lv[LVL];
-  if (3 * L.nodes() >= (int64_t(1) << 31) || g.nelem >= (int64_t(1) << 31)) {
+  if ((int64_t)sizeof(T) * 3 * L.nodes() >= (int64_t(1) << 32) ||
+      (int64_t)sizeof(T) * g.nelem >= (int64_t(1) << 32)) {
     set_error(TOP3D_ERR_ARG, "grid too large for 32-bit offsets (%lld nodes)",
               (long long)L.nodes());
     return 0;
