@@ -861,15 +861,19 @@ static int launch_tile(const DevGrid &g, int mode, const T *E, const T *in, T *o
   constexpr bool kF32Fine = sizeof(T) == 4 && LVL == 0;
   if (cached_nthr[diag] != nthr) {
     const void *kfn;
+    // the FP32 fine level sizes every mode's x-march for two workgroups per
+    // CU, the diagonal-forming sweep's occupancy (a third, which the
+    // residual's registers and LDS would allow, shortened the march: 43.1
+    // against 38.5 us per residual at 256x128x128)
     if constexpr (kF32Fine)
-      kfn = diag ? (const void *)k_tile_sweep_f32<kModeJacobiDiag, false>
-                 : (const void *)k_tile_sweep_f32<kModeJacobi, false>;
+      kfn = (const void *)k_tile_sweep_f32<kModeJacobiDiag, false>;
     else if constexpr (LVL == 0)
       kfn = diag ? (const void *)k_tile_sweep<T, LVL, kModeJacobiDiag, false>
                  : (const void *)k_tile_sweep<T, LVL, kModeJacobi, false>;
     else
       kfn = (const void *)k_tile_sweep<T, LVL, kModeJacobi, false>;
-    cached_slots[diag] = wg_per_cu(kfn, bs, shm);
+    cached_slots[diag] =
+        wg_per_cu(kfn, bs, kF32Fine ? sizeof(T) * 2 * 11 * (size_t)nthr : shm);
     cached_nthr[diag] = nthr;
   }
   tile_x(s, L, nx_fixed, (int64_t)cached_slots[diag] * device_cus());
