@@ -95,11 +95,12 @@ def host_description():
     return {"nproc": os.cpu_count(), "usable_cpus": usable, "cpu_model": model}
 
 
-def cpu_baseline(nelx, nely, nelz, nl, gpu_same):
+def cpu_baseline(nelx, nely, nelz, nl, gpu_same, small_gpu):
     """FP64 oracle on the host: the first design iteration of the cantilever
     (uniform density 0.2, cold start; state solve with its set-up and CG,
     volume, sensitivity, filter gradient, OC, change, filter), run whole and
-    timed, not extrapolated."""
+    timed, not extrapolated; then the reference's small cantilevers beside
+    the GPU's times for them."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     host = host_description()
@@ -112,6 +113,7 @@ def cpu_baseline(nelx, nely, nelz, nl, gpu_same):
     t0 = time.perf_counter()
     recs, _, _ = pyoracle.top3d(nelx, nely, nelz, nl, 1)
     sec = time.perf_counter() - t0
+    small = small_config_cpu(pyoracle, small_gpu, threads)
     return {"value": sec, "unit": "s/design-iter", "cores": threads, "kind": "port",
             "sample": (f"FP64 oracle (oracle/top3d_oracle.c, OpenMP {threads} threads) running "
                        f"the first design iteration of the same {nelx}x{nely}x{nelz} nl={nl} "
@@ -122,7 +124,60 @@ def cpu_baseline(nelx, nely, nelz, nl, gpu_same):
                        f"(FP64) for that same design iteration"),
             "host": host, "omp_num_threads": threads,
             "compliance": recs[0]["c"], "cg_iters": recs[0]["cgiter"] + 1,
-            "gpu_same_iteration": gpu_same}
+            "gpu_same_iteration": gpu_same, "small_configs": small}
+
+
+SMALL_CONFIGS = (("48x24x24, 3 levels", (48, 24, 24, 3), 5),
+                 ("128x64x64, 4 levels", (128, 64, 64, 4), 2))
+
+
+def small_config_gpu(top3d_amd, torch):
+    """The reference's own small cantilevers (-x 12 -y 6 -z 6 -l 3 and
+    -x 16 -y 8 -z 8 -l 4): the first design iterations run whole on the GPU
+    in both forms, on a grid whose kernels are loaded (one design iteration
+    first, untimed)."""
+    out = {}
+    for name, shape, iters in SMALL_CONFIGS:
+        for form in ("fp64", "mixed"):
+            g = top3d_amd.Grid(*shape, precision=form)
+            top3d_amd.design_loop(g, 1)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            recs, _ = top3d_amd.design_loop(g, iters)
+            torch.cuda.synchronize()
+            out[(name, form)] = ((time.perf_counter() - t0) / len(recs),
+                                 [r["cgiter"] + 1 for r in recs])
+            g.close()
+    return out
+
+
+def small_config_cpu(pyoracle, gpu, threads):
+    """The same design iterations with the FP64 oracle on the host, timed
+    around the oracle's whole top3d call (its grid set-up included; the GPU
+    times exclude creating the grid), with all threads and, for the smallest
+    case, one thread."""
+    import ctypes
+    try:
+        gomp = ctypes.CDLL("libgomp.so.1")
+    except OSError:
+        gomp = None
+    rows = []
+    for name, shape, iters in SMALL_CONFIGS:
+        for nthr in ((threads, 1) if shape[0] == 48 and gomp else (threads,)):
+            if gomp:
+                gomp.omp_set_num_threads(nthr)
+            t0 = time.perf_counter()
+            recs, _, _ = pyoracle.top3d(*shape, iters)
+            sec = (time.perf_counter() - t0) / len(recs)
+            g64, cg64 = gpu.get((name, "fp64"), (float("nan"), []))
+            g32, _ = gpu.get((name, "mixed"), (float("nan"), []))
+            rows.append({"workload": f"cantilever {name}", "design_iters": len(recs),
+                         "threads": nthr, "cpu_s_per_design_iter": sec,
+                         "gpu_fp64_s_per_design_iter": g64, "gpu_mixed_s_per_design_iter": g32,
+                         "cg_iters_cpu": [r["cgiter"] + 1 for r in recs], "cg_iters_gpu_fp64": cg64})
+    if gomp:
+        gomp.omp_set_num_threads(threads)
+    return rows
 
 
 def first_iteration_gpu(top3d_amd, torch, g):
@@ -296,6 +351,7 @@ def main():
             gpu_same["fp64_s"], _ = first_iteration_gpu(top3d_amd, torch, g64)
         g64.close()
     g.close()
+    small_gpu = small_config_gpu(top3d_amd, torch) if args.cpu_baseline else {}
 
     c5 = None
     if args.c5:
@@ -307,7 +363,7 @@ def main():
         if not gpu_same:
             gpu_same = {"mixed_s": float("nan"), "fp64_s": float("nan")}
         print("[bench] CPU baseline (oracle) ...", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(nelx, nely, nelz, nl, gpu_same)
+        cpu = cpu_baseline(nelx, nely, nelz, nl, gpu_same, small_gpu)
 
     out = {
         "metric": "MGCG solve s/design-iter at 256x128x128, mixed vs FP64 @rtol 1e-5; matvec GB/s",
