@@ -65,47 +65,6 @@ void launch_dot(const T *a, const T *b, int64_t n, double *partials, double *res
 
 // Device-resident CG scalars (layout shared with the solver):
 //   s[0] rho, s[1] rho_old, s[2] p.q, s[3] r.r, s[4] b.b
-// p = z (first) or p = (rho/rho_old) p + z  (stencil_solvers.c:350-362)
-__global__ void k_pupdate(int64_t n, const double *__restrict__ s, double *__restrict__ p,
-                          const double *__restrict__ z, int first) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if (first) {
-    p[i] = z[i];
-  } else {
-    const double beta = s[0] / s[1];
-    p[i] = beta * p[i] + z[i];
-  }
-}
-
-void launch_pupdate(int64_t n, const double *scal, double *p, const double *z, int first) {
-  const int bs = 256;
-  hipLaunchKernelGGL(k_pupdate, dim3((n + bs - 1) / bs), dim3(bs), 0, stream(), n, scal, p,
-                     z, first);
-}
-
-// u += alpha p; r -= alpha q with alpha = rho / p.q, and rho_old = rho
-// (stencil_solvers.c:366-377); only s[0] and s[2] are read here, so thread 0
-// may store s[1].
-__global__ void k_axpy_cg(int64_t n, double *__restrict__ s, double *__restrict__ u,
-                          double *__restrict__ r, const double *__restrict__ p,
-                          const double *__restrict__ q) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double rho = s[0];
-  const double alpha = rho / s[2];
-  u[i] += alpha * p[i];
-  r[i] -= alpha * q[i];
-  if (i == 0) s[1] = rho;
-}
-
-void launch_axpy_cg(int64_t n, double *scal, double *u, double *r, const double *p,
-                    const double *q) {
-  const int bs = 256;
-  hipLaunchKernelGGL(k_axpy_cg, dim3((n + bs - 1) / bs), dim3(bs), 0, stream(), n, scal, u,
-                     r, p, q);
-}
-
 // ---- fused CG recurrence ---------------------------------------------------
 // The CG scalars are reduced from per-workgroup partial sums by every block
 // of the consuming kernel, in one fixed order, so no separate reduction

@@ -238,8 +238,6 @@ void launch_coarse_apply(const DevGrid &g, int l, const CoarseFactor &cf, const 
 template <typename T>
 void launch_dot(const T *a, const T *b, int64_t n, double *partials, double *result,
                 double *host = nullptr);
-void launch_axpy_cg(int64_t n, double *scal, double *u, double *r, const double *p,
-                    const double *q);
 template <typename TO, typename TI>
 void launch_convert(int64_t n, const TI *in, TO *out);
 template <typename T>
@@ -248,8 +246,6 @@ template <typename T>
 void launch_sub(int64_t n, const T *a, const T *b, T *out);  // out = a - b
 template <typename T>
 void launch_add(int64_t n, const T *a, T *out);               // out += a
-void launch_pupdate(int64_t n, const double *scal, double *p, const double *z,
-                    int first);
 // fused CG steps (partial sums consumed by the next kernel, kRedGrid = 1024
 // partials for launch_dot_rz and the r.r partials)
 // (r and z in the same precision: the FP32 copy of r in the mixed form)
