@@ -180,11 +180,11 @@ void launch_dot_rz(const TZ *r, const TZ *z, int64_t n, double *partials) {
                      partials);
 }
 template <typename TZ>
-void launch_pupdate_fused(int64_t n, const double *partials, double *scal, double *p,
+void launch_pupdate_fused(int64_t n, const double *partials, int npart, double *scal, double *p,
                           const TZ *z, int first, double *u) {
   // 2048 blocks: 8 waves per SIMD on 256 CUs for this streaming update
   hipLaunchKernelGGL((k_pupdate_fused<TZ>), dim3(2 * kRedGrid), dim3(kRedBlock), 0, stream(), n,
-                     partials, kRedGrid, scal, p, z, first, u);
+                     partials, npart, scal, p, z, first, u);
 }
 void launch_u_update(int64_t n, const double *scal, double *u, const double *p) {
   hipLaunchKernelGGL(k_u_update, dim3(2 * kRedGrid), dim3(kRedBlock), 0, stream(), n, scal, u,
@@ -200,9 +200,9 @@ void launch_axpy_fused(int64_t n, const double *pq_part, int npart, double *scal
 }
 template void launch_dot_rz<double>(const double *, const double *, int64_t, double *);
 template void launch_dot_rz<float>(const float *, const float *, int64_t, double *);
-template void launch_pupdate_fused<double>(int64_t, const double *, double *, double *,
+template void launch_pupdate_fused<double>(int64_t, const double *, int, double *, double *,
                                            const double *, int, double *);
-template void launch_pupdate_fused<float>(int64_t, const double *, double *, double *,
+template void launch_pupdate_fused<float>(int64_t, const double *, int, double *, double *,
                                           const float *, int, double *);
 
 template <typename TO, typename TI>

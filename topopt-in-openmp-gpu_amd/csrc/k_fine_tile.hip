@@ -404,7 +404,8 @@ __device__ __forceinline__ void tile_sweep_body(
   const uint32_t e00c = elem_ok ? (uint32_t)k * a.few + j : 0u;
   const uint32_t noff[2][2] = {{0u, (uint32_t)a.wy}, {1u, (uint32_t)a.wy + 1u}};  // [py][pz]
   const uint32_t nown = (uint32_t)(k + 1) * a.wy + (uint32_t)(j + 1);  // owned node in a plane
-  double dacc = 0.0;  // apply mode with dotp: partial sum of in . out
+  // with dotp: partial sum of in . out (apply) or b . out (Jacobi sweeps)
+  double dacc = 0.0;
   // kDiag: moduli of the 4 elements of the previous slab at the owned node
   T es_prev = 0;
 
@@ -593,6 +594,7 @@ __device__ __forceinline__ void tile_sweep_body(
             if (dotp) dacc += (double)uc * (double)o[c];
           } else if (MODE == kModeJacobi || kDiag) {
             o[c] = uc + omega * dd[c] * (bb[c] - kv);
+            if (dotp) dacc += (double)bb[c] * (double)o[c];
           } else {
             o[c] = bb[c] - kv;
           }
@@ -622,7 +624,7 @@ __device__ __forceinline__ void tile_sweep_body(
 #pragma unroll
       for (int c = 0; c < 3; c++) Fa[f][c] = Fb[f][c];
   }
-  if (MODE == kModeApply && dotp) {  // one partial per workgroup, fixed order
+  if (MODE != kModeResidual && dotp) {  // one partial per workgroup, fixed order
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) dacc += __shfl_down(dacc, off, 64);
     __syncthreads();
@@ -933,6 +935,18 @@ int launch_fine_apply_dot(const DevGrid &g, const double *E, const double *p, do
   return launch_tile<double, 0>(g, kModeApply, E, p, q, nullptr, nullptr, 0.0, bcsym, partials,
                                 cap);
 }
+
+template <typename T>
+int launch_fine_tile_sweep_dot(const DevGrid &g, const T *E, const T *in, T *out, const T *b,
+                               const T *invD, T omega, int bcsym, double *partials, int cap) {
+  return launch_tile<T, 0>(g, kModeJacobiDiag, E, in, out, b, invD, omega, bcsym, partials, cap);
+}
+template int launch_fine_tile_sweep_dot<float>(const DevGrid &, const float *, const float *,
+                                               float *, const float *, const float *, float, int,
+                                               double *, int);
+template int launch_fine_tile_sweep_dot<double>(const DevGrid &, const double *, const double *,
+                                                double *, const double *, const double *, double,
+                                                int, double *, int);
 
 template <typename T>
 void launch_level1_tile(const DevGrid &g, int mode, const T *E, const T *in, T *out,

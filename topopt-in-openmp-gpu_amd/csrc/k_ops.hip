@@ -185,6 +185,25 @@ void launch_fine(const DevGrid &g, int mode, const T *E, const T *in, T *out,
                        b, invD, omega, g.fixmask[0], bcsym);
 }
 
+// A fine Jacobi sweep that also writes per-workgroup partial sums of b . out
+// (at most cap of them); returns their count, or 0 when the sweep ran
+// without them (node-centric kernel, or more workgroups than cap).
+template <typename T>
+int launch_fine_sweep_dot(const DevGrid &g, const T *E, const T *in, T *out, const T *b,
+                          const T *invD, T omega, int bcsym, double *partials, int cap) {
+  if (use_node_kernel()) {
+    launch_fine<T>(g, 3, E, in, out, b, invD, omega, bcsym);
+    return 0;
+  }
+  return launch_fine_tile_sweep_dot<T>(g, E, in, out, b, invD, omega, bcsym, partials, cap);
+}
+template int launch_fine_sweep_dot<float>(const DevGrid &, const float *, const float *, float *,
+                                          const float *, const float *, float, int, double *,
+                                          int);
+template int launch_fine_sweep_dot<double>(const DevGrid &, const double *, const double *,
+                                           double *, const double *, const double *, double, int,
+                                           double *, int);
+
 // ---------------------------------------------------------------------------
 // Level-1 operator with the 8 sub-cell matrices and fine moduli
 // (stencil_methods.c:436-519), one thread per coarse node.

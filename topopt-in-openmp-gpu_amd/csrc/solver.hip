@@ -246,11 +246,17 @@ T *SolverState::smooth_from_zero(int l, const T *b, T *zb, T *tb, bool zero_done
   return cur;  // == zb
 }
 
+// The last fine sweep of a preconditioner application also writes the
+// partial sums of r . z for the CG (rz_parts of them, 0 if it could not).
 template <typename T>
 T *SolverState::smooth(int l, const T *b, T *cur, T *oth) {
   const bool rec = l == 0 && nswp > 0 && sweep_timing_begin();
   for (int s = 0; s < nswp; s++) {
-    op<T>(l, 1, cur, oth, b);
+    if (l == 0 && s == nswp - 1)
+      rz_parts = launch_fine_sweep_dot<T>(*g, young_t<T>(), cur, oth, b, invd<T>(0), (T)kOmega,
+                                          bcsym, cgpart, kPartRZ);
+    else
+      op<T>(l, 1, cur, oth, b);
     std::swap(cur, oth);
   }
   sweep_timing_end(rec, nswp);
@@ -288,6 +294,7 @@ void SolverState::vcycle(int l, bool zero_done) {
 // z = M r.  Mixed form: the V-cycle reads the FP32 copy of r written by the
 // CG update and leaves z in FP32, which the CG kernels read directly.
 void SolverState::precondition() {
+  rz_parts = 0;
   if (!mixed) {
     vcycle<double>(0);  // r -> z (aliases)
     return;
@@ -343,13 +350,16 @@ int SolverState::solve(const double *x, const double *b, double *u, double tol, 
   for (iter = 0; iter < max_iter; iter++) {
     if (!pending) precondition();
     pending = false;
-    // rho = r.z; p = z + (rho / rho_old) p
+    // rho = r.z (partials from the V-cycle's last fine sweep, else a dot
+    // kernel); p = z + (rho / rho_old) p
+    int nrz = rz_parts;
     if (mixed) {
-      launch_dot_rz<float>(v32[0][kR], v32[0][kZ], n0, part_rz);
-      launch_pupdate_fused<float>(n0, part_rz, scal, p, v32[0][kZ], iter == 0, u);
+      if (nrz == 0) launch_dot_rz<float>(v32[0][kR], v32[0][kZ], n0, part_rz);
+      launch_pupdate_fused<float>(n0, part_rz, nrz ? nrz : kPartRZ, scal, p, v32[0][kZ],
+                                  iter == 0, u);
     } else {
-      launch_dot_rz<double>(r, z, n0, part_rz);
-      launch_pupdate_fused<double>(n0, part_rz, scal, p, z, iter == 0, u);
+      if (nrz == 0) launch_dot_rz<double>(r, z, n0, part_rz);
+      launch_pupdate_fused<double>(n0, part_rz, nrz ? nrz : kPartRZ, scal, p, z, iter == 0, u);
     }
     // q = K p with the partial sums of p.q
     int npq = launch_fine_apply_dot(*g, E64, p, q, bcsym, part_pq, kPartPQ);

@@ -39,6 +39,7 @@ struct SolverState {
   double *scal = nullptr, *partials = nullptr, *hscal = nullptr;
   double *cgpart = nullptr;  // partial sums of r.z, p.q, r.r
   double *hscal_dev = nullptr;  // device address of hscal
+  int rz_parts = 0;  // r.z partials written by the last preconditioner application
   hipEvent_t ev = nullptr;
   // The assembled levels and the coarse factorization are built on `aux`,
   // concurrently with the start of the solve on the library stream, which

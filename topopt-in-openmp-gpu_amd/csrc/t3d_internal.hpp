@@ -253,7 +253,7 @@ template <typename TZ>
 void launch_dot_rz(const TZ *r, const TZ *z, int64_t n, double *partials);
 // p update; from the second iteration on also the previous u += alpha p
 template <typename TZ>
-void launch_pupdate_fused(int64_t n, const double *partials, double *scal, double *p,
+void launch_pupdate_fused(int64_t n, const double *partials, int npart, double *scal, double *p,
                           const TZ *z, int first, double *u);
 // u += alpha p of the last CG update (alpha from scal)
 void launch_u_update(int64_t n, const double *scal, double *u, const double *p);
@@ -263,6 +263,14 @@ void launch_axpy_fused(int64_t n, const double *pq_part, int npart, double *scal
                        double *host_rr);
 // FP64 fine apply q = K p with per-workgroup partials of p.q; returns their
 // count (0 if `cap` is too small)
+// fine Jacobi sweep (diagonal from the moduli) with per-workgroup partial
+// sums of b . out; returns their count (0: none written)
+template <typename T>
+int launch_fine_tile_sweep_dot(const DevGrid &g, const T *E, const T *in, T *out, const T *b,
+                               const T *invD, T omega, int bcsym, double *partials, int cap);
+template <typename T>
+int launch_fine_sweep_dot(const DevGrid &g, const T *E, const T *in, T *out, const T *b,
+                          const T *invD, T omega, int bcsym, double *partials, int cap);
 int launch_fine_apply_dot(const DevGrid &g, const double *E, const double *p, double *q,
                           int bcsym, double *partials, int cap);
 void launch_copy_fixed(const DevGrid &g, int l, const double *in, double *out);
