@@ -346,6 +346,8 @@ int SolverState::solve(const double *x, const double *b, double *u, double tol, 
   int iter = 0;
   double relres = 0.0;
   bool pending = false;  // a V-cycle for iteration `iter` is already queued
+  // relative residuals of the two previous iterations (< 0: not yet known)
+  double res1 = -1.0, res2 = -1.0;
   double *part_rz = cgpart, *part_pq = cgpart + kPartRZ, *part_rr = part_pq + kPartPQ;
   for (iter = 0; iter < max_iter; iter++) {
     if (!pending) precondition();
@@ -372,9 +374,15 @@ int SolverState::solve(const double *x, const double *b, double *u, double tol, 
     launch_axpy_fused(n0, part_pq, npq, scal, r, q, mixed ? v32[0][kR] : nullptr, part_rr,
                       scal + 3, hscal_dev + 3);
     hipEventRecord(ev, stream());
-    // queue the next V-cycle while the residual norm travels to the host; if
-    // this iteration converged its result is simply not used
-    if (iter + 1 < max_iter) {
+    // queue the next V-cycle while the residual norm travels to the host
+    // (if this iteration converges, its result is not used), unless the last
+    // contraction predicts convergence: then the host waits for the norm
+    // (a few tens of microseconds with the device idle when the prediction
+    // is wrong) instead of a whole V-cycle running after the last iteration
+    bool queue_next = iter + 1 < max_iter;
+    if (queue_next && res1 > 0.0 && res2 > 0.0 && res1 * (res1 / res2) < 2.0 * tol)
+      queue_next = false;
+    if (queue_next) {
       precondition();
       pending = true;
     }
@@ -384,6 +392,8 @@ int SolverState::solve(const double *x, const double *b, double *u, double tol, 
     }
     if (iter == 0) bnorm = std::sqrt(hscal[4]);
     relres = std::sqrt(hscal[3]) / bnorm;
+    res2 = res1;
+    res1 = relres;
     if (final_iter) *final_iter = iter;
     if (final_res) *final_res = (float)relres;
     if (!std::isfinite(relres)) {
