@@ -243,6 +243,7 @@ void release_dev_grid(const struct gridContext *gc) {
     dfree(g->KEf[i]);
   }
   dfree(g->KE2);
+  dfree(g->KE3);
   for (int i = 0; i < 8; i++) dfree(g->scratch[i]);
   delete g;
 }
@@ -321,6 +322,13 @@ static DevGrid *build_dev_grid(struct gridContext *gc, int nl) {
     const size_t cnt = 576 * 64;
     g->KE2 = (double *)dalloc(sizeof(double) * cnt);
     T3D_HIP(hipMemcpyAsync(g->KE2, gc->precomputedKE[2], sizeof(double) * cnt,
+                           hipMemcpyHostToDevice, stream()));
+    T3D_HIP(hipStreamSynchronize(stream()));
+  }
+  if (nl >= 4) {  // level-3 sub-cell matrices: level-3 element matrices as one GEMM
+    const size_t cnt = 576 * 512;
+    g->KE3 = (double *)dalloc(sizeof(double) * cnt);
+    T3D_HIP(hipMemcpyAsync(g->KE3, gc->precomputedKE[3], sizeof(double) * cnt,
                            hipMemcpyHostToDevice, stream()));
     T3D_HIP(hipStreamSynchronize(stream()));
   }

@@ -11,6 +11,10 @@
 // then gathered into a 27-point node stencil with symmetric elimination of
 // the fixed DOFs (stencil_assembly.c:226-259), which serves the smoother
 // levels and the dense coarsest matrix.
+#include <mutex>
+
+#include <rocblas/rocblas.h>
+
 #include "t3d_internal.hpp"
 
 namespace t3d {
@@ -171,6 +175,50 @@ void launch_elem_level2(const DevGrid &g, const double *E, double *Ke2) {
   }
   hipLaunchKernelGGL((k_galerkin<0>), dim3(nel), dim3(576), 0, stream(), L.nelx,
                      L.nely, L.nelz, 0, 0, g.ewy, g.ewz, g.KE[1], E, nullptr, Ke2);
+}
+
+// Moduli of the 512 fine elements of each level-3 element, row e (compact
+// level-3 element index (I*nelz + K)*nely + J), column f = 64 ii + 8 kk + jj
+// (the sub-cell order of getKEsubspace, local_matrix.c:283).
+__global__ __launch_bounds__(256) void k_gather_level3(int nelx, int nely, int nelz, int few,
+                                                       int fez, const double *__restrict__ E,
+                                                       double *__restrict__ Eg) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)nelx * nely * nelz * 512) return;
+  const int64_t e = t >> 9;
+  const int f = (int)(t & 511);
+  const int J = (int)(e % nely), K = (int)((e / nely) % nelz), I = (int)(e / ((int64_t)nely * nelz));
+  const int ii = f >> 6, kk = (f >> 3) & 7, jj = f & 7;
+  Eg[t] = E[((int64_t)(8 * I + ii + 1) * fez + (8 * K + kk + 1)) * few + (8 * J + jj + 1)];
+}
+
+// one rocBLAS handle per device, created on first use
+static rocblas_handle blas_handle() {
+  static std::mutex mu;
+  static rocblas_handle handles[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!handles[dev] && rocblas_create_handle(&handles[dev]) != rocblas_status_success)
+    handles[dev] = nullptr;
+  return handles[dev];
+}
+
+bool launch_elem_level3(const DevGrid &g, const double *E, double *Eg, double *Ke3) {
+  const Level &L = g.lv[3];
+  const int64_t nel = L.elements();
+  if (!g.KE3 || nel <= 0 || nel > (int64_t)INT32_MAX / 576) return false;
+  rocblas_handle h = blas_handle();
+  if (!h || rocblas_set_stream(h, stream()) != rocblas_status_success) return false;
+  const int64_t nth = nel * 512;
+  hipLaunchKernelGGL(k_gather_level3, dim3((unsigned)((nth + 255) / 256)), dim3(256), 0,
+                     stream(), L.nelx, L.nely, L.nelz, g.ewy, g.ewz, E, Eg);
+  // row-major Ke3 [nel x 576] = Eg [nel x 512] KE_3 [512 x 576], i.e. in
+  // column-major terms Ke3^T (576 x nel) = KE_3^T (576 x 512) Eg^T (512 x nel)
+  const double one = 1.0, zero = 0.0;
+  return rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, 576, (rocblas_int)nel,
+                       512, &one, g.KE3, 576, Eg, 512, &zero, Ke3, 576) ==
+         rocblas_status_success;
 }
 
 void launch_elem_galerkin(const DevGrid &g, int l, const double *Kchild, double *Kparent) {

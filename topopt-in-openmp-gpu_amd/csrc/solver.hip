@@ -70,6 +70,7 @@ bool SolverState::allocate(DevGrid *grid, int nlev) {
   dfree(v64[0][kZ]);
   v64[0][kZ] = z;
   const Level &C = g->lv[nl - 1];
+  if (nl >= 4) Eg3 = alloc_t<double>(g->lv[3].elements() * 512);
   ncoarse = (int)(3 * C.interior_nodes());
   bandwidth = coarse_bandwidth(C);
   if (!coarse.allocate(ncoarse, bandwidth, true, C.nx)) return false;
@@ -81,17 +82,22 @@ bool SolverState::allocate(DevGrid *grid, int nlev) {
     return false;
   if (!T3D_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming))) return false;
   if (!T3D_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking)) ||
+      !T3D_HIP(hipStreamCreateWithFlags(&aux2, hipStreamNonBlocking)) ||
       !T3D_HIP(hipEventCreateWithFlags(&ev_base, hipEventDisableTiming)) ||
       !T3D_HIP(hipEventCreateWithFlags(&ev_stencil, hipEventDisableTiming)) ||
       !T3D_HIP(hipEventCreateWithFlags(&ev_coarse, hipEventDisableTiming)) ||
-      !T3D_HIP(hipEventCreateWithFlags(&ev_done, hipEventDisableTiming)))
+      !T3D_HIP(hipEventCreateWithFlags(&ev_done, hipEventDisableTiming)) ||
+      !T3D_HIP(hipEventCreateWithFlags(&ev_done2, hipEventDisableTiming)))
     return false;
+  for (int l = 2; l < nl - 1; l++)
+    if (!T3D_HIP(hipEventCreateWithFlags(&ev_ke[l], hipEventDisableTiming))) return false;
   return T3D_HIP(hipStreamSynchronize(stream())) && r && p && q && z;
 }
 
 void SolverState::release() {
   hipStreamSynchronize(stream());
   if (aux) hipStreamSynchronize(aux);
+  if (aux2) hipStreamSynchronize(aux2);
   auto f = [](void *ptr) { dfree(ptr); };
   for (int l = 0; l < nl; l++) {
     f(invD64[l]);
@@ -106,15 +112,20 @@ void SolverState::release() {
     f(S64[l]);
     f(S32[l]);
   }
-  f(r); f(p); f(q); f(z); f(E64); f(E32);
+  f(r); f(p); f(q); f(z); f(E64); f(E32); f(Eg3);
   coarse.release();
   f(scal); f(partials); f(cgpart);
   if (hscal) hipHostFree(hscal);
   if (ev) hipEventDestroy(ev);
-  for (hipEvent_t e : {ev_base, ev_stencil, ev_coarse, ev_done})
+  for (hipEvent_t e : {ev_base, ev_stencil, ev_coarse, ev_done, ev_done2})
     if (e) hipEventDestroy(e);
+  for (hipEvent_t &e : ev_ke) {
+    if (e) hipEventDestroy(e);
+    e = nullptr;
+  }
   if (aux) hipStreamDestroy(aux);
-  aux = nullptr;
+  if (aux2) hipStreamDestroy(aux2);
+  aux = aux2 = nullptr;
   for (hipEvent_t e : sw_ev) hipEventDestroy(e);
   sw_ev.clear();
   magic = 0;
@@ -138,30 +149,34 @@ bool SolverState::setup_main(const double *x) {
   return true;
 }
 
-// The assembled levels and the coarse factorization, queued on `aux` after
-// the moduli (ev_base).
+// The assembled levels and the coarse factorization after the moduli
+// (ev_base).  On `aux`: the element matrices level by level from level 3,
+// whose matrices come straight from the fine moduli (one library GEMM), so
+// that the coarse chain does not wait for level 2, then the coarsest level's
+// stencil, dense matrix and factor.  On `aux2`: the level-2 element matrices
+// and the smoother levels' stencils and inverse diagonals, each after its
+// level's element matrices (ev_ke[l]).  Without the level-3 GEMM (fewer than
+// 4 levels, or the library unavailable) `aux` starts from level 2 and builds
+// level 3 by the Galerkin product.
 bool SolverState::setup_aux() {
+  bool direct3 = false;
   {
     StreamScope on_aux(aux);
     if (!T3D_HIP(hipStreamWaitEvent(aux, ev_base, 0))) return false;
-    for (int l = 2; l < nl; l++) {
+    if (nl >= 4) direct3 = launch_elem_level3(*g, E64, Eg3, Ke[3]);
+    for (int l = direct3 ? 3 : 2; l < nl; l++) {
       if (l == 2)
         launch_elem_level2(*g, E64, Ke[2]);
-      else
+      else if (!(direct3 && l == 3))
         launch_elem_galerkin(*g, l, Ke[l - 1], Ke[l]);
-      // smoother levels: the symmetric half layout (and its FP32 copy for
-      // the mixed V-cycle); the coarsest level: the full stencil
-      const bool smoother = l < nl - 1;
-      launch_node_stencil(*g, l, Ke[l], S64[l], 1, (mixed && smoother) ? S32[l] : nullptr,
-                          smoother);
-      if (smoother)
-        launch_invdiag_half<double>(*g, l, S64[l], invD64[l]);
-      else
+      if (l < nl - 1) {
+        if (!T3D_HIP(hipEventRecord(ev_ke[l], aux))) return false;
+      } else {
+        // the coarsest level: the full stencil for its dense assembly
+        launch_node_stencil(*g, l, Ke[l], S64[l], 1, nullptr, false);
         launch_invdiag_stencil<double>(*g, l, S64[l], invD64[l]);
-      if (mixed && smoother) launch_convert<float, double>(g->lv[l].ndof, invD64[l], invD32[l]);
-      if (l == nl - 2) hipEventRecord(ev_stencil, aux);
+      }
     }
-    if (nl < 4) hipEventRecord(ev_stencil, aux);  // no assembled smoother level
     launch_stencil_to_dense(*g, nl - 1, S64[nl - 1], coarse);
     coarse.factor_async(mixed);
     if (!T3D_HIP(hipEventRecord(ev_coarse, aux))) return false;
@@ -169,6 +184,23 @@ bool SolverState::setup_aux() {
     // it from E; it is still filled in for callers of the SolverDataBuffer
     if (fine_sweep_forms_diagonal()) launch_invdiag_mf<double>(*g, 0, E64, invD64[0]);
     if (!T3D_HIP(hipEventRecord(ev_done, aux))) return false;
+  }
+  {
+    StreamScope on_aux2(aux2);
+    if (direct3) {
+      if (!T3D_HIP(hipStreamWaitEvent(aux2, ev_base, 0))) return false;
+      launch_elem_level2(*g, E64, Ke[2]);
+    }
+    // smoother levels: the symmetric half layout (and its FP32 copy for the
+    // mixed V-cycle)
+    for (int l = 2; l < nl - 1; l++) {
+      if (!(direct3 && l == 2) && !T3D_HIP(hipStreamWaitEvent(aux2, ev_ke[l], 0))) return false;
+      launch_node_stencil(*g, l, Ke[l], S64[l], 1, mixed ? S32[l] : nullptr, true);
+      launch_invdiag_half<double>(*g, l, S64[l], invD64[l]);
+      if (mixed) launch_convert<float, double>(g->lv[l].ndof, invD64[l], invD32[l]);
+    }
+    if (!T3D_HIP(hipEventRecord(ev_stencil, aux2))) return false;
+    if (!T3D_HIP(hipEventRecord(ev_done2, aux2))) return false;
   }
   wait_stencil = wait_coarse = true;
   return true;
@@ -334,6 +366,7 @@ int SolverState::solve(const double *x, const double *b, double *u, double tol, 
   // the coarse set-up is queued while the library stream computes r
   if (!setup_aux()) {
     hipStreamSynchronize(aux);
+    hipStreamSynchronize(aux2);
     hipStreamSynchronize(stream());
     wait_stencil = wait_coarse = false;
     stats.status = top3d_last_error();
@@ -404,8 +437,9 @@ int SolverState::solve(const double *x, const double *b, double *u, double tol, 
     if (relres < tol) break;
   }
   if (max_iter > 0) launch_u_update(n0, scal, u, p);  // the last iteration's u += alpha p
-  join_coarse();  // the work on `aux` has finished before the solve returns
+  join_coarse();  // the work on `aux` and `aux2` has finished before the solve returns
   hipStreamWaitEvent(stream(), ev_done, 0);
+  hipStreamWaitEvent(stream(), ev_done2, 0);
   if (!T3D_HIP(hipStreamSynchronize(stream()))) status = TOP3D_ERR_HIP;
   if (status == TOP3D_OK) {
     const int rc = coarse.check_info();

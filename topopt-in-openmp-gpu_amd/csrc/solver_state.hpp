@@ -29,6 +29,7 @@ struct SolverState {
   float *invD32[kMaxLevels] = {};
   // assembled levels: element matrices and 27-point stencils
   double *Ke[kMaxLevels] = {};
+  double *Eg3 = nullptr;  // moduli gathered per level-3 element (library GEMM operand)
   double *S64[kMaxLevels] = {};
   float *S32[kMaxLevels] = {};
   // coarsest level: Cholesky factor and W = L^{-1}
@@ -45,8 +46,13 @@ struct SolverState {
   // concurrently with the start of the solve on the library stream, which
   // waits for ev_stencil before its first sweep on level 2 and for ev_coarse
   // before its first coarse solve.
-  hipStream_t aux = nullptr;
+  // The element matrices of the assembled levels and the coarse factor run
+  // on `aux`; the smoother levels' stencils run on `aux2` as their element
+  // matrices become ready (ev_ke[l]), off the coarse factor's path.
+  hipStream_t aux = nullptr, aux2 = nullptr;
   hipEvent_t ev_base = nullptr, ev_stencil = nullptr, ev_coarse = nullptr, ev_done = nullptr;
+  hipEvent_t ev_done2 = nullptr;
+  hipEvent_t ev_ke[kMaxLevels] = {};
   bool wait_stencil = false, wait_coarse = false;
   void join_stencil();
   void join_coarse();

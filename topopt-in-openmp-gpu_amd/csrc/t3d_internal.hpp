@@ -72,6 +72,7 @@ struct DevGrid {
   double *KE[2] = {};                  // device KE_0 (576), KE_1 (8*576)
   float *KEf[2] = {};
   double *KE2 = nullptr;               // device KE_2 (64*576), nl >= 3
+  double *KE3 = nullptr;               // device KE_3 (512*576), nl >= 4
   // 3x3 blocks of KE_0 in the per-component Walsh-Hadamard basis of the
   // element's nodal values (valid if wh0)
   bool wh0 = false;
@@ -158,6 +159,10 @@ void launch_invdiag_stencil(const DevGrid &g, int l, const double *S, T *invD);
 
 // coarse element matrices and stencils (FP64)
 void launch_elem_level2(const DevGrid &g, const double *E, double *Ke2);
+// Level-3 element matrices Ke3[e] = sum over the element's 512 fine elements
+// f of E_f KE_3[f], as a library FP64 GEMM on the moduli gathered into Eg
+// (level-3 elements x 512); false if it cannot run (no KE_3, library error).
+bool launch_elem_level3(const DevGrid &g, const double *E, double *Eg, double *Ke3);
 void launch_elem_galerkin(const DevGrid &g, int l, const double *Kchild,
                           double *Kparent);
 // S32: FP32 copy of S written alongside (optional).  half: the symmetric
