@@ -244,6 +244,7 @@ void release_dev_grid(const struct gridContext *gc) {
   }
   dfree(g->KE2);
   dfree(g->KE3);
+  for (int l = 0; l < kMaxLevels; l++) dfree(g->KEdiag[l]);
   for (int i = 0; i < 8; i++) dfree(g->scratch[i]);
   delete g;
 }
@@ -329,6 +330,18 @@ static DevGrid *build_dev_grid(struct gridContext *gc, int nl) {
     const size_t cnt = 576 * 512;
     g->KE3 = (double *)dalloc(sizeof(double) * cnt);
     T3D_HIP(hipMemcpyAsync(g->KE3, gc->precomputedKE[3], sizeof(double) * cnt,
+                           hipMemcpyHostToDevice, stream()));
+    T3D_HIP(hipStreamSynchronize(stream()));
+  }
+  // diagonals of the grid's own sub-cell matrices for the matrix-free
+  // inverse diagonals (levels whose 24 x 8^l table stays small)
+  for (int l = 0; l < nl && l < 5; l++) {
+    const size_t ncell = (size_t)1 << (3 * l);
+    std::vector<double> dg(24 * ncell);
+    for (size_t c = 0; c < ncell; c++)
+      for (int r = 0; r < 24; r++) dg[24 * c + r] = gc->precomputedKE[l][576 * c + 25 * r];
+    g->KEdiag[l] = (double *)dalloc(sizeof(double) * dg.size());
+    T3D_HIP(hipMemcpyAsync(g->KEdiag[l], dg.data(), sizeof(double) * dg.size(),
                            hipMemcpyHostToDevice, stream()));
     T3D_HIP(hipStreamSynchronize(stream()));
   }

@@ -999,6 +999,43 @@ bool launch_fine_jacobi_zero(const DevGrid &g, const TB *b, const T *E, T omega,
                      L.wz, g.ewy, g.ewz, b, E, g.fixmask[0], B, omega, out);
   return true;
 }
+// Inverse diagonal of the fine operator from the moduli (KE_0's diagonal is
+// one value per component): 1 / (KE[c][c] sum of the node's 8 element
+// moduli) on real nodes, 1 at fixed DOFs, 0 elsewhere (the values of
+// assembleInvertedMatrixDiagonalSubspace_halo, stencil_methods.c:693-773).
+template <typename T>
+__global__ __launch_bounds__(256) void k_invdiag_fine(int nx, int ny, int nz, int wy, int wz,
+                                                      int few, int fez, const T *__restrict__ E,
+                                                      const uint8_t *__restrict__ fix,
+                                                      Blocks<T> B, T *__restrict__ invD) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= wy * wz) return;
+  const int j = t % wy, k = t / wy, i = blockIdx.y;
+  const int64_t node = (int64_t)i * wz * wy + t;
+  T *po = invD + 3 * node;
+  if (!(i >= 1 && i <= nx && j >= 1 && j <= ny && k >= 1 && k <= nz)) {
+    po[0] = po[1] = po[2] = T(0);
+    return;
+  }
+  auto slab = [&](int ie) {
+    const int64_t e = ((int64_t)ie * fez + (k - 1)) * few + (j - 1);
+    return ((E[e] + E[e + 1]) + E[e + few]) + E[e + few + 1];
+  };
+  const T rs = T(1) / (slab(i - 1) + slab(i));
+  const uint8_t m = fix[node];
+#pragma unroll
+  for (int c = 0; c < 3; c++) po[c] = ((m >> c) & 1) ? T(1) : rs * B.ikd[c];
+}
+
+bool launch_fine_invdiag(const DevGrid &g, const double *E, double *invD) {
+  if (!tile_fine_diag_available()) return false;
+  const Level &L = g.lv[0];
+  dim3 bs(256), gs((L.wy * L.wz + 255) / 256, L.wx);
+  hipLaunchKernelGGL((k_invdiag_fine<double>), gs, bs, 0, stream(), L.nx, L.ny, L.nz, L.wy,
+                     L.wz, g.ewy, g.ewz, E, g.fixmask[0], g_coef.b64, invD);
+  return true;
+}
+
 template bool launch_fine_jacobi_zero<double, double>(const DevGrid &, const double *,
                                                       const double *, double, double *);
 template bool launch_fine_jacobi_zero<float, float>(const DevGrid &, const float *,

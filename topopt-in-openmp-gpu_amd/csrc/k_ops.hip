@@ -701,7 +701,8 @@ __global__ void k_invdiag_mf(LevelArgs a, int ncell, const double *__restrict__ 
   for (int c = 0; c < 3; c++) invD[3 * node + c] = (T)(1.0 / (((m >> c) & 1) ? 1.0 : d[c]));
 }
 
-// diagonal entries of the sub-cell matrices of level l (24 per sub-cell),
+// diagonal entries of the sub-cell matrices of level l (24 per sub-cell) at
+// the reference's nu = 0.3, for levels whose table the grid does not keep;
 // built once per level and kept for the life of the process
 static double *kediag_for(int l) {
   static double *cache[kMaxLevels] = {};
@@ -724,7 +725,7 @@ static double *kediag_for(int l) {
 template <typename T>
 void launch_invdiag_mf(const DevGrid &g, int l, const double *E, T *invD) {
   const Level &L = g.lv[l];
-  const double *kd = kediag_for(l);
+  const double *kd = g.KEdiag[l] ? g.KEdiag[l] : kediag_for(l);
   dim3 bs(64, 4, 1), gs((L.wy + 63) / 64, (L.wz + 3) / 4, L.wx);
   hipLaunchKernelGGL((k_invdiag_mf<T>), gs, bs, 0, stream(), g.args(l), L.ncell, kd, E,
                      g.fixmask[l], invD);

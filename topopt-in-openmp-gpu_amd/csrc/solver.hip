@@ -182,7 +182,8 @@ bool SolverState::setup_aux() {
     if (!T3D_HIP(hipEventRecord(ev_coarse, aux))) return false;
     // the fine inverse diagonal is not read by the fine sweeps when they form
     // it from E; it is still filled in for callers of the SolverDataBuffer
-    if (fine_sweep_forms_diagonal()) launch_invdiag_mf<double>(*g, 0, E64, invD64[0]);
+    if (fine_sweep_forms_diagonal() && !launch_fine_invdiag(*g, E64, invD64[0]))
+      launch_invdiag_mf<double>(*g, 0, E64, invD64[0]);
     if (!T3D_HIP(hipEventRecord(ev_done, aux))) return false;
   }
   {

@@ -73,6 +73,7 @@ struct DevGrid {
   float *KEf[2] = {};
   double *KE2 = nullptr;               // device KE_2 (64*576), nl >= 3
   double *KE3 = nullptr;               // device KE_3 (512*576), nl >= 4
+  double *KEdiag[kMaxLevels] = {};     // diagonals of the level-l sub-cell matrices (24 each)
   // 3x3 blocks of KE_0 in the per-component Walsh-Hadamard basis of the
   // element's nodal values (valid if wh0)
   bool wh0 = false;
@@ -276,6 +277,9 @@ int launch_fine_tile_sweep_dot(const DevGrid &g, const T *E, const T *in, T *out
 template <typename T>
 int launch_fine_sweep_dot(const DevGrid &g, const T *E, const T *in, T *out, const T *b,
                           const T *invD, T omega, int bcsym, double *partials, int cap);
+// fine inverse diagonal from the moduli (false: the element matrix lacks the
+// per-component diagonal this needs)
+bool launch_fine_invdiag(const DevGrid &g, const double *E, double *invD);
 int launch_fine_apply_dot(const DevGrid &g, const double *E, const double *p, double *q,
                           int bcsym, double *partials, int cap);
 void launch_copy_fixed(const DevGrid &g, int l, const double *in, double *out);
