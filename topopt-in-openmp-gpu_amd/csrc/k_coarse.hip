@@ -179,17 +179,19 @@ void launch_elem_level2(const DevGrid &g, const double *E, double *Ke2) {
 
 // Moduli of the 512 fine elements of each level-3 element, row e (compact
 // level-3 element index (I*nelz + K)*nely + J), column f = 64 ii + 8 kk + jj
-// (the sub-cell order of getKEsubspace, local_matrix.c:283).
+// (the sub-cell order of getKEsubspace, local_matrix.c:283).  One thread per
+// fine element, fine y fastest: the reads are contiguous rows of E, the
+// writes runs of 8 moduli.
 __global__ __launch_bounds__(256) void k_gather_level3(int nelx, int nely, int nelz, int few,
                                                        int fez, const double *__restrict__ E,
                                                        double *__restrict__ Eg) {
+  const int fny = 8 * nely, fnz = 8 * nelz;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)nelx * nely * nelz * 512) return;
-  const int64_t e = t >> 9;
-  const int f = (int)(t & 511);
-  const int J = (int)(e % nely), K = (int)((e / nely) % nelz), I = (int)(e / ((int64_t)nely * nelz));
-  const int ii = f >> 6, kk = (f >> 3) & 7, jj = f & 7;
-  Eg[t] = E[((int64_t)(8 * I + ii + 1) * fez + (8 * K + kk + 1)) * few + (8 * J + jj + 1)];
+  if (t >= (int64_t)8 * nelx * fnz * fny) return;
+  const int fj = (int)(t % fny), fk = (int)((t / fny) % fnz), fi = (int)(t / ((int64_t)fny * fnz));
+  const int64_t e = ((int64_t)(fi >> 3) * nelz + (fk >> 3)) * nely + (fj >> 3);
+  const int f = 64 * (fi & 7) + 8 * (fk & 7) + (fj & 7);
+  Eg[e * 512 + f] = E[((int64_t)(fi + 1) * fez + (fk + 1)) * few + (fj + 1)];
 }
 
 // one rocBLAS handle per device, created on first use

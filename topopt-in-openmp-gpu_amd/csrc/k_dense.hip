@@ -76,9 +76,9 @@ __device__ inline void store_acc(double *X, const double4_t acc[2][2], int qr, i
 // ---------------------------------------------------------------------------
 // Cholesky and inverse of one 64x64 SPD tile in LDS, 256 threads (4 waves),
 // blocked by 16: per diagonal block kb
-//   (a) wave 0, lanes 0..15 (one row each, cross-lane reads): the 16x16
-//       Cholesky L_D (reciprocal square roots refined by Newton steps) and
-//       its inverse X_D by forward substitution;
+//   (a) wave 0, lanes 0..15 (one row each; each pivot column broadcast
+//       through LDS): the 16x16 Cholesky L_D (reciprocal square roots
+//       refined by Newton steps) and its inverse X_D by forward substitution;
 //   (b) the panel below: P <- P X_D^T on MFMA;
 //   (c) the trailing lower triangle: C -= P P^T as 16x16 MFMA tiles;
 // then the off-diagonal 16x16 blocks of the inverse by distance on MFMA.
@@ -106,17 +106,21 @@ __device__ inline double rsqrt_refined(double x) {
 }
 
 __device__ __forceinline__ void factor_tile(double *T, double *X, int *info) {
+  __shared__ double col[16];  // the current pivot column of a 16x16 block
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
   const int li = lane & 15, lk = lane >> 4;
   for (int e = t; e < kNB * kNB; e += 256) X[(e >> 6) * kLd + (e & 63)] = 0.0;
   __syncthreads();
   for (int kb = 0; kb < 4; kb++) {
     const int d0 = 16 * kb;
-    if (t < 16) {  // (a): one row per lane, broadcasts by v_readlane
+    if (t < 16) {  // (a): one row per lane
       const int r = t;
+      // the row's entries right of the diagonal are updated as well and
+      // dropped when the row is stored: a lane only ever reads the column
+      // values of lanes below the current pivot, which are exact
       double v[16], inv[16];
 #pragma unroll
-      for (int c = 0; c < 16; c++) v[c] = (c <= r) ? T[(d0 + r) * kLd + d0 + c] : 0.0;
+      for (int c = 0; c < 16; c++) v[c] = T[(d0 + r) * kLd + d0 + c];
 #pragma unroll
       for (int j = 0; j < 16; j++) {
         double piv = readlane_d(v[j], j);
@@ -127,24 +131,31 @@ __device__ __forceinline__ void factor_tile(double *T, double *X, int *info) {
         const double ij = rsqrt_refined(piv);
         inv[j] = ij;  // 1 / L_jj
         const double lrj = (r == j) ? piv * ij : v[j] * ij;
-        v[j] = (r >= j) ? lrj : 0.0;
+        v[j] = lrj;
+        if (j < 15) {
+          // column j to every lane through LDS (the wave's LDS accesses
+          // complete in order: each read sees this write, and the next
+          // pivot's write follows these reads)
+          col[r] = lrj;
 #pragma unroll
-        for (int c = j + 1; c < 16; c++) {
-          const double lcj = readlane_d(lrj, c);
-          if (c <= r) v[c] -= lrj * lcj;
+          for (int c = j + 1; c < 16; c++) v[c] -= lrj * col[c];
         }
       }
 #pragma unroll
-      for (int c = 0; c < 16; c++) T[(d0 + r) * kLd + d0 + c] = v[c];
+      for (int c = 0; c < 16; c++) T[(d0 + r) * kLd + d0 + c] = (c <= r) ? v[c] : 0.0;
       // column r of X_D = L_D^{-1}: forward substitution with the rows of
-      // L_D broadcast from their lanes
+      // L_D read back from LDS (the same address for every lane), two
+      // partial sums to halve the dependent chain
       double x[16];
 #pragma unroll
       for (int i = 0; i < 16; i++) {
-        double acc = (i == r) ? 1.0 : 0.0;
+        double a0 = (i == r) ? 1.0 : 0.0, a1 = 0.0;
 #pragma unroll
-        for (int m = 0; m < i; m++) acc -= readlane_d(v[m], i) * x[m];
-        x[i] = acc * inv[i];
+        for (int m = 0; m < i; m += 2) {
+          a0 -= T[(d0 + i) * kLd + d0 + m] * x[m];
+          if (m + 1 < i) a1 -= T[(d0 + i) * kLd + d0 + m + 1] * x[m + 1];
+        }
+        x[i] = (a0 + a1) * inv[i];
         X[(d0 + i) * kLd + d0 + r] = x[i];
       }
     }
@@ -527,6 +538,18 @@ bool CoarseFactor::allocate(int n_, int bw_, bool f32, int planes) {
   cw64 = (double *)dalloc(sizeof(double) * 2 * (size_t)np);
   pos = (int *)dalloc(sizeof(int) * (size_t)std::max(n, 1));
   ipos = (int *)dalloc(sizeof(int) * (size_t)np);
+  std::vector<int> zt;
+  for (const CholTask &tk : cs.tasks) {
+    zt.push_back(tk.I * cs.nblk + tk.J);
+    for (int q = 0; q < tk.nsrc; q++) {
+      zt.push_back(tk.I * cs.nblk + tk.K[q]);
+      zt.push_back(tk.J * cs.nblk + tk.K[q]);
+    }
+  }
+  std::sort(zt.begin(), zt.end());
+  zt.erase(std::unique(zt.begin(), zt.end()), zt.end());
+  nztiles = (int)zt.size();
+  dztiles = (int *)dalloc(sizeof(int) * std::max<size_t>(1, zt.size()));
   ntask = (int)cs.tasks.size();
   dtasks = (CholTask *)dalloc(sizeof(CholTask) * std::max<size_t>(1, cs.tasks.size()));
   dcolinfo = (int2 *)dalloc(sizeof(int) * cs.colinfo.size());
@@ -538,7 +561,7 @@ bool CoarseFactor::allocate(int n_, int bw_, bool f32, int planes) {
     cw32 = (float *)dalloc(sizeof(float) * 2 * (size_t)np);
   }
   if (!(A && L && W && WT && Dinv && info && cw64 && pos && ipos && dtasks && dcolinfo &&
-        drows && dklist && (!f32 || (W32 && WT32 && cw32))))
+        drows && dklist && dztiles && (!f32 || (W32 && WT32 && cw32))))
     return false;
   launch_first = cs.launch_first;
   launch_count = cs.launch_count;
@@ -547,6 +570,7 @@ bool CoarseFactor::allocate(int n_, int bw_, bool f32, int planes) {
     return bytes == 0 || T3D_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
   };
   return up(pos, cs.pos.data(), sizeof(int) * n) && up(ipos, hipos.data(), sizeof(int) * np) &&
+         up(dztiles, zt.data(), sizeof(int) * zt.size()) &&
          up(dtasks, cs.tasks.data(), sizeof(CholTask) * cs.tasks.size()) &&
          up(dcolinfo, cs.colinfo.data(), sizeof(int) * cs.colinfo.size()) &&
          up(drows, cs.rows.data(), sizeof(int) * cs.rows.size()) &&
@@ -558,8 +582,10 @@ void CoarseFactor::release() {
   for (void *p : {(void *)A, (void *)L, (void *)W, (void *)WT, (void *)Dinv, (void *)info,
                   (void *)cw64, (void *)W32, (void *)WT32, (void *)cw32, (void *)pos,
                   (void *)ipos, (void *)dtasks, (void *)dcolinfo, (void *)drows,
-                  (void *)dklist})
+                  (void *)dklist, (void *)dztiles})
     if (p) dfree(p);
+  dztiles = nullptr;
+  nztiles = 0;
   A = L = W = WT = Dinv = cw64 = nullptr;
   W32 = WT32 = cw32 = nullptr;
   info = pos = ipos = dklist = nullptr;
@@ -568,8 +594,19 @@ void CoarseFactor::release() {
   drows = nullptr;
 }
 
+// zero the listed 64x64 tiles of an np x np matrix, one workgroup per tile
+__global__ __launch_bounds__(256) void k_zero_tiles(double *__restrict__ A, int np, int nblk,
+                                                    const int *__restrict__ tiles) {
+  const int tile = tiles[blockIdx.x];
+  const int64_t r0 = (int64_t)(tile / nblk) * kNB, c0 = (int64_t)(tile % nblk) * kNB;
+  for (int e = threadIdx.x; e < kNB * kNB; e += 256) A[(r0 + (e >> 6)) * np + c0 + (e & 63)] = 0.0;
+}
+
+// A = 0 on the tiles the factorization reads, identity on the padding
 void CoarseFactor::begin_assembly() {
-  launch_fill<double>((int64_t)np * np, A, 0.0);
+  if (nztiles > 0)
+    hipLaunchKernelGGL(k_zero_tiles, dim3(nztiles), dim3(256), 0, stream(), A, np, nblk,
+                       dztiles);
   hipLaunchKernelGGL(k_pad_identity, dim3((np + 255) / 256), dim3(256), 0, stream(), np, ipos,
                      A);
 }

@@ -214,12 +214,16 @@ struct CoarseFactor {
   int2 *dcolinfo = nullptr;
   int4 *drows = nullptr;
   int *dklist = nullptr;
+  // the tiles I * nblk + J the factorization reads (targets and sources of
+  // its tasks): the only part of A that assembly has to clear
+  int *dztiles = nullptr;
+  int nztiles = 0;
   // planes > 1 with n divisible by planes: nested dissection over the planes
   // (n / planes consecutive DOFs each, coupled to adjacent planes only);
   // otherwise banded with half bandwidth bw
   bool allocate(int n, int bw, bool f32, int planes);
   void release();
-  void begin_assembly();  // A = 0 with identity padding
+  void begin_assembly();  // A = 0 on the tiles read by the factorization, identity padding
   // TOP3D_OK or TOP3D_ERR_NOT_SPD / TOP3D_ERR_HIP; f32out: W32 and WT32 for
   // the FP32 apply instead of the FP64 WT
   int factor(bool f32out = false);
