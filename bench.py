@@ -301,12 +301,18 @@ def main():
     solve_s = sum(r["solve_s"] for r in recs) / len(recs)
     cg_mixed = [r["cgiter"] + 1 for r in recs]
 
-    # kernel timings on the mixed run's buffers (HIP events, library stream)
+    # kernel timings on the mixed run's buffers (HIP events, library stream):
+    # back to back on the same buffers (operand sets under 256 MiB then stay
+    # in the memory-side cache), and "cold", each launch after a 1 GiB copy
+    # that flushes that cache, as inside a solve
     kernels = {}
     for name in ("fine_sweep_f32", "level1_sweep_f32", "fine_apply_f64", "fine_sweep_f64"):
         ms, by = g.time_kernel(name, args.kernel_reps)
+        cms, _ = g.time_kernel(name, args.kernel_reps, cold=True)
         kernels[name] = {"avg_ms": ms, "bytes": by, "GB_s": by / (ms * 1e-3) / 1e9,
-                         "hbm_frac": by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+                         "hbm_frac": by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "cold_avg_ms": cms,
+                         "cold_hbm_frac": by / (cms * 1e-3) / 1e9 / HBM_PEAK_GBS}
     # dominant kernel of the mixed solve: the FP32 fine sweep (7 per V-cycle),
     # timed live over the timed design iterations
     live = recs[0]["sweep_live"]

@@ -390,6 +390,12 @@ int top3d_compliance_backward(const struct gpuGrid *gpu_gc, double rmin,
  *   4 level-1 damped-Jacobi sweep, FP64
  *   5 fine operator, FP64, with the per-workgroup partial sums of p.q (the
  *     form the CG iteration launches)
+ * which | 16: before each launch, stream 1 GiB through a scratch buffer so
+ * that the kernel finds none of its operands in the memory-side cache (as
+ * inside a solve), and time each launch alone.  Without it the launches run
+ * back to back on the same buffers, and an operand set smaller than that
+ * cache (the FP64 operator's p, q and moduli: 212 MB at 256x128x128) is
+ * served from it.
  * Outputs the mean milliseconds per launch and the algorithmic HBM bytes of
  * one launch (true nodes and elements only: vectors read/written once,
  * moduli read once).                                                         */

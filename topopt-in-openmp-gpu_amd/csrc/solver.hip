@@ -556,10 +556,15 @@ int top3d_sweep_timing(struct SolverDataBuffer *data, int enable, double *avg_ms
   return TOP3D_OK;
 }
 
+// `which` flag: time each launch alone, with the memory-side cache flushed
+static constexpr int kTimeCold = 16;
+
 int top3d_time_kernel(struct SolverDataBuffer *data, int which, int reps, double *avg_ms,
                       double *bytes) {
   SolverState *s = solver_state(data);
   if (!s) return top3d_last_error();
+  const bool cold = (which & kTimeCold) != 0;
+  which &= ~kTimeCold;
   if (reps < 1 || which < 0 || which > 5) {
     set_error(TOP3D_ERR_ARG, "top3d_time_kernel: which=%d reps=%d", which, reps);
     return TOP3D_ERR_ARG;
@@ -614,12 +619,39 @@ int top3d_time_kernel(struct SolverDataBuffer *data, int which, int reps, double
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
-  hipEventRecord(e0, stream());
-  for (int r = 0; r < reps; r++) launch(r);
-  hipEventRecord(e1, stream());
-  if (!T3D_HIP(hipEventSynchronize(e1))) return TOP3D_ERR_HIP;
   float ms = 0.f;
-  hipEventElapsedTime(&ms, e0, e1);
+  if (!cold) {
+    hipEventRecord(e0, stream());
+    for (int r = 0; r < reps; r++) launch(r);
+    hipEventRecord(e1, stream());
+    if (!T3D_HIP(hipEventSynchronize(e1))) return TOP3D_ERR_HIP;
+    hipEventElapsedTime(&ms, e0, e1);
+  } else {
+    // before each launch, a device-to-device copy of kFlushBytes moves
+    // 2 x kFlushBytes through the memory-side cache (256 MiB on MI355X), so
+    // the kernel finds none of its operands there, as in a solve whose
+    // other vectors pass between two launches; each launch is timed alone
+    constexpr size_t kFlushBytes = size_t(512) << 20;
+    char *flush = nullptr;
+    bool ok = T3D_HIP(hipMalloc(&flush, 2 * kFlushBytes));
+    for (int r = 0; ok && r < reps; r++) {
+      hipMemcpyAsync(flush + (r & 1) * kFlushBytes, flush + (~r & 1) * kFlushBytes, kFlushBytes,
+                     hipMemcpyDeviceToDevice, stream());
+      hipEventRecord(e0, stream());
+      launch(r);
+      hipEventRecord(e1, stream());
+      ok = T3D_HIP(hipEventSynchronize(e1));
+      float m = 0.f;
+      if (ok) hipEventElapsedTime(&m, e0, e1);
+      ms += m;
+    }
+    if (flush) hipFree(flush);
+    if (!ok) {
+      hipEventDestroy(e0);
+      hipEventDestroy(e1);
+      return TOP3D_ERR_HIP;
+    }
+  }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   if (avg_ms) *avg_ms = ms / reps;

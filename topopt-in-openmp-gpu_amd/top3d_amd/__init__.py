@@ -455,10 +455,12 @@ class Grid:
     KERNELS = {"fine_sweep_f32": 0, "fine_sweep_f64": 1, "fine_apply_f64": 2,
                "level1_sweep_f32": 3, "level1_sweep_f64": 4, "fine_apply_dot_f64": 5}
 
-    def time_kernel(self, name, reps=20):
-        """(mean ms per launch, algorithmic bytes per launch) of a hot kernel."""
+    def time_kernel(self, name, reps=20, cold=False):
+        """(mean ms per launch, algorithmic bytes per launch) of a hot kernel.
+        cold=True flushes the memory-side cache before each launch."""
         ms, by = c_double(), c_double()
-        rc = self.lib.top3d_time_kernel(ctypes.byref(self.data), self.KERNELS[name], reps,
+        which = self.KERNELS[name] | (16 if cold else 0)
+        rc = self.lib.top3d_time_kernel(ctypes.byref(self.data), which, reps,
                                         ctypes.byref(ms), ctypes.byref(by))
         if rc:
             check("top3d_time_kernel")
