@@ -245,7 +245,7 @@ void release_dev_grid(const struct gridContext *gc) {
   dfree(g->KE2);
   dfree(g->KE3);
   for (int l = 0; l < kMaxLevels; l++) dfree(g->KEdiag[l]);
-  for (int i = 0; i < 8; i++) dfree(g->scratch[i]);
+  for (int i = 0; i < DevGrid::kScratchSlots; i++) dfree(g->scratch[i]);
   delete g;
 }
 

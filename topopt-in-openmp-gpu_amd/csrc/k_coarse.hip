@@ -179,19 +179,36 @@ void launch_elem_level2(const DevGrid &g, const double *E, double *Ke2) {
 
 // Moduli of the 512 fine elements of each level-3 element, row e (compact
 // level-3 element index (I*nelz + K)*nely + J), column f = 64 ii + 8 kk + jj
-// (the sub-cell order of getKEsubspace, local_matrix.c:283).  One thread per
-// fine element, fine y fastest: the reads are contiguous rows of E, the
-// writes runs of 8 moduli.
+// (the sub-cell order of getKEsubspace, local_matrix.c:283).  One workgroup
+// per kG3Run level-3 elements consecutive in y: it reads the 64 (ii, kk)
+// rows of 8 kG3Run fine moduli (contiguous in E), and writes the elements'
+// rows of Eg, which are consecutive, as one contiguous run from LDS (rows
+// padded to 520 so that the row-wise LDS stores of the reads spread over
+// the banks).
+constexpr int kG3Run = 4;
+constexpr int kG3Ld = 520;
+
 __global__ __launch_bounds__(256) void k_gather_level3(int nelx, int nely, int nelz, int few,
                                                        int fez, const double *__restrict__ E,
                                                        double *__restrict__ Eg) {
-  const int fny = 8 * nely, fnz = 8 * nelz;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)8 * nelx * fnz * fny) return;
-  const int fj = (int)(t % fny), fk = (int)((t / fny) % fnz), fi = (int)(t / ((int64_t)fny * fnz));
-  const int64_t e = ((int64_t)(fi >> 3) * nelz + (fk >> 3)) * nely + (fj >> 3);
-  const int f = 64 * (fi & 7) + 8 * (fk & 7) + (fj & 7);
-  Eg[e * 512 + f] = E[((int64_t)(fi + 1) * fez + (fk + 1)) * few + (fj + 1)];
+  __shared__ double buf[kG3Run * kG3Ld];
+  const int nrun = (nely + kG3Run - 1) / kG3Run;
+  const int64_t ik = blockIdx.x / nrun;  // I * nelz + K
+  const int J0 = kG3Run * (int)(blockIdx.x % nrun);
+  const int K = (int)(ik % nelz), I = (int)(ik / nelz);
+  const int nj = min(kG3Run, nely - J0);
+  for (int q = threadIdx.x; q < 64 * 8 * kG3Run; q += blockDim.x) {
+    const int row = q / (8 * kG3Run), c = q % (8 * kG3Run);  // row = 8 ii + kk
+    if (c < 8 * nj) {
+      const int fi = 8 * I + (row >> 3), fk = 8 * K + (row & 7), fj = 8 * J0 + c;
+      buf[(c >> 3) * kG3Ld + 8 * row + (c & 7)] =
+          E[((int64_t)(fi + 1) * fez + (fk + 1)) * few + (fj + 1)];
+    }
+  }
+  __syncthreads();
+  double *dst = Eg + (ik * nely + J0) * 512;
+  for (int q = threadIdx.x; q < 512 * nj; q += blockDim.x)
+    dst[q] = buf[(q >> 9) * kG3Ld + (q & 511)];
 }
 
 // one rocBLAS handle per device, created on first use
@@ -212,9 +229,9 @@ bool launch_elem_level3(const DevGrid &g, const double *E, double *Eg, double *K
   if (!g.KE3 || nel <= 0 || nel > (int64_t)INT32_MAX / 576) return false;
   rocblas_handle h = blas_handle();
   if (!h || rocblas_set_stream(h, stream()) != rocblas_status_success) return false;
-  const int64_t nth = nel * 512;
-  hipLaunchKernelGGL(k_gather_level3, dim3((unsigned)((nth + 255) / 256)), dim3(256), 0,
-                     stream(), L.nelx, L.nely, L.nelz, g.ewy, g.ewz, E, Eg);
+  const int64_t nwg = (int64_t)L.nelx * L.nelz * ((L.nely + kG3Run - 1) / kG3Run);
+  hipLaunchKernelGGL(k_gather_level3, dim3((unsigned)nwg), dim3(256), 0, stream(), L.nelx,
+                     L.nely, L.nelz, g.ewy, g.ewz, E, Eg);
   // row-major Ke3 [nel x 576] = Eg [nel x 512] KE_3 [512 x 576], i.e. in
   // column-major terms Ke3^T (576 x nel) = KE_3^T (576 x 512) Eg^T (512 x nel)
   const double one = 1.0, zero = 0.0;

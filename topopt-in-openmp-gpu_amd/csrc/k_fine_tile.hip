@@ -391,15 +391,14 @@ __device__ __forceinline__ void tile_sweep_body(
   const bool last_y = (ty == a.TY - 1), last_z = (tz == a.TZ - 1);
   const bool own_ok = elem_ok && (!last_y || j + 1 == a.nely + 1) &&
                       (!last_z || k + 1 == a.nelz + 1);
-  // level 1: the element is real (its fine moduli exist)
-  const bool real_yz = elem_ok && j >= 1 && k >= 1;
   // 32-bit offsets: the host checks sizeof(T) * 3 * nodes < 2^32
   const uint32_t sx = (uint32_t)a.wy * (uint32_t)a.wz;
   const uint32_t n00 = (uint32_t)k * a.wy + j;  // node (j, k) within a plane
   // Threads whose element column lies outside the grid read the halo node
   // (0, 0) of each plane and the halo element (i, 0, 0) instead, both in
   // bounds; that element's modulus is 0, so their products are 0 and no
-  // branch guards the loads.
+  // branch guards the loads.  (Level 1: E holds the transformed moduli of
+  // k_level1_moduli, 8 per element slot, with strides few, fez of its own.)
   const uint32_t n00c = elem_ok ? n00 : 0u;
   const uint32_t e00c = elem_ok ? (uint32_t)k * a.few + j : 0u;
   const uint32_t noff[2][2] = {{0u, (uint32_t)a.wy}, {1u, (uint32_t)a.wy + 1u}};  // [py][pz]
@@ -500,37 +499,28 @@ __device__ __forceinline__ void tile_sweep_body(
         }
       }
     } else {
-      T Eo[8];
-#pragma unroll
-      for (int o = 0; o < 8; o++) Eo[o] = 0;
-      if (real_yz && i >= 1 && i <= a.nelx) {
-#pragma unroll
-        for (int o = 0; o < 8; o++) {
-          const uint32_t fe = ((uint32_t)(2 * i - 1 + (o & 1)) * a.fez + (2 * k - 1 + (o >> 2))) *
-                                  a.few + (2 * j - 1 + ((o >> 1) & 1));
-          Eo[o] = *at32(E, fe);
-        }
-      }
-      // Ehat = per-element 8-point transform of the moduli
+      // Ehat: the per-element 8-point transform of the element's fine moduli,
+      // formed once per set of moduli (k_level1_moduli); zero for halo
+      // elements, which threads outside the grid read
       T Eh[8];
-      {
-        T p1[8], p2[8];
+      const uint32_t eh = 8u * ((uint32_t)i * (uint32_t)(a.fez * a.few) + e00c);
+      if constexpr (sizeof(T) == 4) {
+        typedef float f4 __attribute__((ext_vector_type(4)));
+        const f4 *pe = reinterpret_cast<const f4 *>(at32(E, eh));
+        const f4 lo = pe[0], hi = pe[1];
 #pragma unroll
-        for (int o = 0; o < 8; o += 2) {
-          p1[o] = Eo[o] + Eo[o + 1];
-          p1[o + 1] = Eo[o] - Eo[o + 1];
+        for (int o = 0; o < 4; o++) {
+          Eh[o] = lo[o];
+          Eh[o + 4] = hi[o];
         }
+      } else {
+        typedef double d2 __attribute__((ext_vector_type(2)));
+        const d2 *pe = reinterpret_cast<const d2 *>(at32(E, eh));
 #pragma unroll
-        for (int o = 0; o < 8; o += 4)
-#pragma unroll
-          for (int q = 0; q < 2; q++) {
-            p2[o + q] = p1[o + q] + p1[o + q + 2];
-            p2[o + q + 2] = p1[o + q] - p1[o + q + 2];
-          }
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-          Eh[q] = p2[q] + p2[q + 4];
-          Eh[q + 4] = p2[q] - p2[q + 4];
+        for (int o = 0; o < 4; o++) {
+          const d2 v = pe[o];
+          Eh[2 * o] = v[0];
+          Eh[2 * o + 1] = v[1];
         }
       }
       element_core_l1<T>(B, Eh, h, g);
@@ -843,8 +833,9 @@ static int launch_tile(const DevGrid &g, int mode, const T *E, const T *in, T *o
     return 0;
   }
   const Level &L = g.lv[LVL];
+  const int64_t nmod = LVL == 0 ? g.nelem : level1_moduli_count(g);
   if ((int64_t)sizeof(T) * 3 * L.nodes() >= (int64_t(1) << 32) ||
-      (int64_t)sizeof(T) * g.nelem >= (int64_t(1) << 32)) {
+      (int64_t)sizeof(T) * nmod >= (int64_t(1) << 32)) {
     set_error(TOP3D_ERR_ARG, "grid too large for 32-bit offsets (%lld nodes)",
               (long long)L.nodes());
     return 0;
@@ -885,8 +876,8 @@ static int launch_tile(const DevGrid &g, int mode, const T *E, const T *in, T *o
   a.nelz = L.nelz;
   a.wy = L.wy;
   a.wz = L.wz;
-  a.few = g.ewy;
-  a.fez = g.ewz;
+  a.few = LVL == 0 ? g.ewy : L.nely + 1;  // level 1: the slots of k_level1_moduli
+  a.fez = LVL == 0 ? g.ewz : L.nelz + 1;
   a.TY = s.TY;
   a.TZ = s.TZ;
   a.NX = s.NX;
@@ -949,10 +940,74 @@ template int launch_fine_tile_sweep_dot<double>(const DevGrid &, const double *,
                                                 int, double *, int);
 
 template <typename T>
-void launch_level1_tile(const DevGrid &g, int mode, const T *E, const T *in, T *out,
+void launch_level1_tile(const DevGrid &g, int mode, const T *Eh, const T *in, T *out,
                         const T *b, const T *invD, T omega, int bcsym) {
-  launch_tile<T, 1>(g, mode, E, in, out, b, invD, omega, bcsym);
+  launch_tile<T, 1>(g, mode, Eh, in, out, b, invD, omega, bcsym);
 }
+
+// Per level-1 element slot (i, k, j), i in 0..nelx+1, k in 0..nelz, j in
+// 0..nely (y fastest): the 8-point Walsh-Hadamard transform of the moduli
+// of its 8 fine elements (sub-cell o: x offset o & 1, y offset (o >> 1) & 1,
+// z offset o >> 2), the form in which the level-1 sweep applies them; zero
+// for the halo slots.  The moduli are fixed for a solve, so this runs once
+// per solve instead of once per element step of every level-1 sweep.
+template <typename T>
+__global__ __launch_bounds__(256) void k_level1_moduli(int nelx, int nely, int nelz, int few,
+                                                       int fez, const T *__restrict__ E,
+                                                       T *__restrict__ Eh) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int sy = nely + 1, sz = nelz + 1;
+  if (t >= (int64_t)(nelx + 2) * sz * sy) return;
+  const int j = (int)(t % sy), k = (int)((t / sy) % sz), i = (int)(t / ((int64_t)sy * sz));
+  T Eo[8];
+#pragma unroll
+  for (int o = 0; o < 8; o++) Eo[o] = 0;
+  if (i >= 1 && i <= nelx && j >= 1 && k >= 1) {
+#pragma unroll
+    for (int o = 0; o < 8; o++) {
+      const int64_t fe = ((int64_t)(2 * i - 1 + (o & 1)) * fez + (2 * k - 1 + (o >> 2))) * few +
+                         (2 * j - 1 + ((o >> 1) & 1));
+      Eo[o] = E[fe];
+    }
+  }
+  T p1[8], p2[8], h[8];
+#pragma unroll
+  for (int o = 0; o < 8; o += 2) {
+    p1[o] = Eo[o] + Eo[o + 1];
+    p1[o + 1] = Eo[o] - Eo[o + 1];
+  }
+#pragma unroll
+  for (int o = 0; o < 8; o += 4)
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      p2[o + q] = p1[o + q] + p1[o + q + 2];
+      p2[o + q + 2] = p1[o + q] - p1[o + q + 2];
+    }
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    h[q] = p2[q] + p2[q + 4];
+    h[q + 4] = p2[q] - p2[q + 4];
+  }
+  T *pe = Eh + 8 * t;
+#pragma unroll
+  for (int o = 0; o < 8; o++) pe[o] = h[o];
+}
+
+int64_t level1_moduli_count(const DevGrid &g) {
+  const Level &L = g.lv[1];
+  return g.nl < 2 ? 0 : 8 * (int64_t)(L.nelx + 2) * (L.nely + 1) * (L.nelz + 1);
+}
+
+template <typename T>
+void launch_level1_moduli(const DevGrid &g, const T *E, T *Eh) {
+  const Level &L = g.lv[1];
+  const int64_t n = level1_moduli_count(g) / 8;
+  if (n <= 0) return;
+  hipLaunchKernelGGL((k_level1_moduli<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     stream(), L.nelx, L.nely, L.nelz, g.ewy, g.ewz, E, Eh);
+}
+template void launch_level1_moduli<double>(const DevGrid &, const double *, double *);
+template void launch_level1_moduli<float>(const DevGrid &, const float *, float *);
 
 // First pre-smoothing sweep of the fine level from z = 0, with the diagonal
 // formed from the moduli as in the kJacobiDiag sweep:

@@ -291,9 +291,16 @@ static bool use_node_level1() {
 
 template <typename T>
 void launch_level1(const DevGrid &g, int mode, const T *E, const T *in, T *out,
-                   const T *b, const T *invD, T omega, int bcsym) {
+                   const T *b, const T *invD, T omega, int bcsym, const T *Eh) {
   if (!use_node_level1() && tile_level1_available()) {
-    launch_level1_tile<T>(g, mode, E, in, out, b, invD, omega, bcsym);
+    if (!Eh) {
+      T *w = (T *)const_cast<DevGrid &>(g).work(kSlotLevel1Moduli,
+                                                sizeof(T) * level1_moduli_count(g));
+      if (!w) return;
+      launch_level1_moduli<T>(g, E, w);
+      Eh = w;
+    }
+    launch_level1_tile<T>(g, mode, Eh, in, out, b, invD, omega, bcsym);
     return;
   }
   const Level &L = g.lv[1];
@@ -794,7 +801,7 @@ void launch_invdiag_stencil(const DevGrid &g, int l, const double *S, T *invD) {
   template void launch_fine<T>(const DevGrid &, int, const T *, const T *, T *,          \
                                const T *, const T *, T, int);                            \
   template void launch_level1<T>(const DevGrid &, int, const T *, const T *, T *,        \
-                                 const T *, const T *, T, int);                          \
+                                 const T *, const T *, T, int, const T *);               \
   template void launch_stencil<T>(const DevGrid &, int, int, const T *, const T *, T *,  \
                                   const T *, const T *, T);                              \
   template void launch_stencil_rows<T>(const DevGrid &, int, int, const T *, const T *,  \

@@ -48,6 +48,7 @@ bool SolverState::allocate(DevGrid *grid, int nlev) {
   z = alloc_t<double>(n0);
   E64 = alloc_t<double>(g->nelem);
   E32 = alloc_t<float>(g->nelem);
+  if (nl >= 2) Eh1 = alloc_t<double>(level1_moduli_count(*g));
   for (int l = 0; l < nl; l++) {
     const int64_t nd = g->lv[l].ndof;
     invD64[l] = alloc_t<double>(nd);
@@ -112,7 +113,7 @@ void SolverState::release() {
     f(S64[l]);
     f(S32[l]);
   }
-  f(r); f(p); f(q); f(z); f(E64); f(E32); f(Eg3);
+  f(r); f(p); f(q); f(z); f(E64); f(E32); f(Eh1); f(Eg3);
   coarse.release();
   f(scal); f(partials); f(cgpart);
   if (hscal) hipHostFree(hscal);
@@ -139,6 +140,13 @@ bool SolverState::setup_main(const double *x) {
   // the second stream needs only E64
   if (!T3D_HIP(hipEventRecord(ev_base, stream()))) return false;
   if (mixed) launch_convert<float, double>(g->nelem, E64, E32);
+  eh1_set = false;
+  if (nl >= 2 && Eh1 && tile_level1_available()) {
+    if (mixed) launch_level1_moduli<float>(*g, E32, (float *)Eh1);
+    else launch_level1_moduli<double>(*g, E64, (double *)Eh1);
+    eh1_f32 = mixed != 0;
+    eh1_set = true;
+  }
   // the fine Jacobi sweeps form their diagonal from E when they can
   const bool fdiag = fine_sweep_forms_diagonal();
   if (!fdiag) launch_invdiag_mf<double>(*g, 0, E64, invD64[0]);
@@ -230,7 +238,7 @@ void SolverState::op(int l, int mode, const T *in, T *out, const T *b) {
   if (l == 0)  // Jacobi sweeps form the diagonal from the moduli
     launch_fine<T>(*g, mode == 1 ? 3 : mode, young_t<T>(), in, out, b, invD, omega, bcsym);
   else if (l == 1)
-    launch_level1<T>(*g, mode, young_t<T>(), in, out, b, invD, omega, bcsym);
+    launch_level1<T>(*g, mode, young_t<T>(), in, out, b, invD, omega, bcsym, eh1_t<T>());
   else
     launch_stencil_half<T>(*g, l, mode, stencil_t<T>(l), in, out, b, invD, omega);
 }
@@ -580,6 +588,15 @@ int top3d_time_kernel(struct SolverDataBuffer *data, int which, int reps, double
     launch_convert<float, double>(g.lv[l].ndof, s->v64[l][SolverState::kD],
                                   s->v32[l][SolverState::kD]);
   }
+  // the level-1 sweeps' transformed moduli, formed before the timed launches
+  const void *eh1 = nullptr;
+  if (which == 3 || which == 4) {
+    void *w = s->g->work(kSlotLevel1Moduli, (which == 3 ? 4 : 8) * level1_moduli_count(g));
+    if (!w) return top3d_last_error();
+    if (which == 3) launch_level1_moduli<float>(g, s->E32, (float *)w);
+    else launch_level1_moduli<double>(g, s->E64, (double *)w);
+    eh1 = w;
+  }
   auto launch = [&](int rep) {
     const bool odd = rep & 1;
     switch (which) {
@@ -605,13 +622,14 @@ int top3d_time_kernel(struct SolverDataBuffer *data, int which, int reps, double
       case 3: {
         float *a = s->v32[1][SolverState::kZ], *b = s->t32[1];
         launch_level1<float>(g, 1, s->E32, odd ? b : a, odd ? a : b, s->v32[1][SolverState::kD],
-                             s->invD32[1], (float)kOmega, s->bcsym);
+                             s->invD32[1], (float)kOmega, s->bcsym, (const float *)eh1);
         break;
       }
       default: {
         double *a = s->v64[1][SolverState::kZ], *b = s->t64[1];
         launch_level1<double>(g, 1, s->E64, odd ? b : a, odd ? a : b,
-                              s->v64[1][SolverState::kD], s->invD64[1], kOmega, s->bcsym);
+                              s->v64[1][SolverState::kD], s->invD64[1], kOmega, s->bcsym,
+                              (const double *)eh1);
       }
     }
   };

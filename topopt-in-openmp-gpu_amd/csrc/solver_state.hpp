@@ -20,6 +20,14 @@ struct SolverState {
   // Young's moduli of the fine elements
   double *E64 = nullptr;
   float *E32 = nullptr;
+  // their per-level-1-element transform for the level-1 sweeps, in the
+  // precision of the last set-up's V-cycle (launch_level1_moduli; sized for
+  // FP64)
+  void *Eh1 = nullptr;
+  bool eh1_f32 = false, eh1_set = false;
+  template <typename T> const T *eh1_t() const {
+    return (eh1_set && eh1_f32 == (sizeof(T) == 4)) ? (const T *)Eh1 : nullptr;
+  }
   // per level: V-cycle vectors r, z, d (FP64 and FP32), ping-pong buffer
   double *v64[kMaxLevels][3] = {};
   float *v32[kMaxLevels][3] = {};

@@ -78,8 +78,9 @@ struct DevGrid {
   // element's nodal values (valid if wh0)
   bool wh0 = false;
   double M0[8][3][3] = {};
-  void *scratch[8] = {};               // lazily allocated work arrays of the API
-  size_t scratch_bytes[8] = {};
+  static constexpr int kScratchSlots = 9;
+  void *scratch[kScratchSlots] = {};   // lazily allocated work arrays of the API
+  size_t scratch_bytes[kScratchSlots] = {};
   LevelArgs args(int l) const;
   void *work(int slot, size_t bytes);  // scratch[slot], grown to >= bytes
 };
@@ -115,19 +116,30 @@ void launch_fine(const DevGrid &g, int mode, const T *E, const T *in, T *out,
 template <typename T>
 void launch_fine_tile(const DevGrid &g, int mode, const T *E, const T *in, T *out,
                       const T *b, const T *invD, T omega, int bcsym);
+// Eh: the transformed moduli of launch_level1_moduli
 template <typename T>
-void launch_level1_tile(const DevGrid &g, int mode, const T *E, const T *in, T *out,
+void launch_level1_tile(const DevGrid &g, int mode, const T *Eh, const T *in, T *out,
                         const T *b, const T *invD, T omega, int bcsym);
+// per level-1 element slot the 8-point transform of its fine moduli (the
+// operand of the level-1 tile sweep); level1_moduli_count values
+template <typename T>
+void launch_level1_moduli(const DevGrid &g, const T *E, T *Eh);
+int64_t level1_moduli_count(const DevGrid &g);
+// DevGrid::work slot of the transformed moduli formed by launch_level1
+// when the caller passes none
+constexpr int kSlotLevel1Moduli = 8;
 bool tile_level1_available();  // the level-1 element matrices have the mirror structure
 bool tile_fine_diag_available();  // the fine tile sweep can form its diagonal from E
 // launch_fine mode 3 forms the diagonal from E (no invD read)
 bool fine_sweep_forms_diagonal();
 // zero the padded y rows (j > nely+1) of a level-0 vector
 void launch_zero_pad_rows(const DevGrid &g, double *out);
-// level-1 matrix-free operator with fine moduli, same modes
+// level-1 matrix-free operator with fine moduli E, same modes; Eh: their
+// transform by launch_level1_moduli (formed here into a DevGrid work slot
+// when null)
 template <typename T>
 void launch_level1(const DevGrid &g, int mode, const T *E, const T *in, T *out,
-                   const T *b, const T *invD, T omega, int bcsym);
+                   const T *b, const T *invD, T omega, int bcsym, const T *Eh = nullptr);
 // assembled 27-point node stencil (levels >= 2), same modes
 template <typename T>
 void launch_stencil(const DevGrid &g, int l, int mode, const T *S, const T *in,
