@@ -418,8 +418,8 @@ void launch_stencil_rows(const DevGrid &g, int l, int mode, const T *S, const T 
                        L.nz, L.wy, L.wz, S, in, out, b, invD, omega);
 }
 
-// Solver variant on the symmetric half layout (launch_node_stencil, half):
-// one thread per (interior node n, row r).  Row r of the block towards an
+// Solver variant on the symmetric half layout (launch_node_stencil, half),
+// per (interior node n, row r).  Row r of the block towards an
 // upper neighbour nb > 13 is stored at n; towards a lower neighbour
 // nb < 13 it is column r of the block that the neighbour n' stores towards
 // n (its upper neighbour 26 - nb), read there when n' is an interior node
@@ -432,59 +432,88 @@ __device__ __forceinline__ T ld_at(const T *base, uint32_t byte_off) {
   return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
 }
 
+// Part of row r of (K in) at node n from the 9 neighbours in node plane
+// i + DX: sum over (dz, dy) in the row-major order of the 27-point stencil.
+template <typename T, int DX>
+__device__ __forceinline__ T half_row_part(const T *__restrict__ Sh, const T *__restrict__ in,
+                                           uint32_t nint, int nx, int ny, int nz, int wy, int wz,
+                                           int i, int j, int k, int r, uint32_t sb,
+                                           uint32_t ub) {
+  T acc = 0;
+#pragma unroll
+  for (int dz = -1; dz <= 1; dz++)
+#pragma unroll
+    for (int dy = -1; dy <= 1; dy++) {
+      const int nbi = (DX + 1) * 9 + (dz + 1) * 3 + (dy + 1);
+      const T *uq = in + 3 * (int64_t)((DX * wz + dz) * wy + dy);  // neighbour q of the node
+      T s0, s1, s2;
+      if (nbi > 13) {
+        const T *s = Sh + (int64_t)((nbi - 14) * 9 + 3 * r) * nint;
+        s0 = ld_at(s, sb);
+        s1 = ld_at(s + nint, sb);
+        s2 = ld_at(s + 2 * (int64_t)nint, sb);
+      } else if (nbi == 13) {  // symmetric centre: C[r][c] = C[min][max]
+        const T *c = Sh + (int64_t)117 * nint;
+        s0 = ld_at(c + (int64_t)half_centre_index(0, r) * nint, sb);
+        s1 = ld_at(c + (int64_t)half_centre_index(r < 1 ? r : 1, r < 1 ? 1 : r) * nint, sb);
+        s2 = ld_at(c + (int64_t)half_centre_index(r, 2) * nint, sb);
+      } else {
+        const bool inside = i + DX >= 0 && i + DX < nx && j + dy >= 0 && j + dy < ny &&
+                            k + dz >= 0 && k + dz < nz;
+        s0 = s1 = s2 = T(0);
+        if (inside) {  // column r of the block neighbour n + dn stores towards n
+          const T *s = Sh + (int64_t)((12 - nbi) * 9 + r) * nint + ((DX * nz + dz) * ny + dy);
+          s0 = ld_at(s, sb);
+          s1 = ld_at(s + 3 * (int64_t)nint, sb);
+          s2 = ld_at(s + 6 * (int64_t)nint, sb);
+        }
+      }
+      acc += s0 * ld_at(uq, ub) + s1 * ld_at(uq + 1, ub) + s2 * ld_at(uq + 2, ub);
+    }
+  return acc;
+}
+
+// Workgroups of 3 waves on 64 consecutive nodes and one row r: wave w sums
+// the neighbours in node plane i + w - 1, and the partial sums meet in LDS
+// (row = (P_-1 + P_0) + P_+1), which the middle wave finishes.  Three waves
+// per (node, row) instead of one thread hold three times the loads in
+// flight on these small, latency-bound levels.
 // The device deals workgroups to its 8 XCDs in turn.  Workgroup w works on
 // (node block, row) pair v = (w % 8) * per + w / 8 of the 3 * nblk pairs,
 // so each XCD sweeps one contiguous eighth of the stencil, the same eighth
 // in every sweep, and can keep it in its own L2 across sweeps.
 template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_stencil_half(
+__global__ __launch_bounds__(192) void k_stencil_half(
     int nx, int ny, int nz, int wy, int wz, const T *__restrict__ Sh,
     const T *__restrict__ in, T *__restrict__ out, const T *__restrict__ b,
     const T *__restrict__ invD, T omega, uint32_t nblk, uint32_t per) {
+  __shared__ T part[2][64];
   const uint32_t v = (blockIdx.x % 8u) * per + blockIdx.x / 8u;
-  if (v >= 3u * nblk) return;
+  if (v >= 3u * nblk) return;  // (uniform in the workgroup)
+  const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
   const uint32_t nint = (uint32_t)nx * ny * nz;
-  const uint32_t n = (v / 3u) * blockDim.x + threadIdx.x;
+  const uint32_t n = (v / 3u) * 64u + (uint32_t)lane;
   const int r = (int)(v % 3u);
-  if (n >= nint) return;
+  const bool ok = n < nint;
   const uint32_t nyz = (uint32_t)ny * nz;
-  const int i = (int)(n / nyz), k = (int)((n - i * nyz) / ny), j = (int)(n - i * nyz - k * ny);
+  const int i = ok ? (int)(n / nyz) : 0, k = ok ? (int)((n - i * nyz) / ny) : 0;
+  const int j = ok ? (int)(n - i * nyz - k * ny) : 0;
   const uint32_t node = ((uint32_t)(i + 1) * wz + (k + 1)) * wy + (j + 1);
   const uint32_t sb = (uint32_t)sizeof(T) * n;         // byte offset in a stencil plane
   const uint32_t ub = (uint32_t)sizeof(T) * 3 * node;  // byte offset of the node's values
   T acc = 0;
-#pragma unroll
-  for (int dx = -1; dx <= 1; dx++)
-#pragma unroll
-    for (int dz = -1; dz <= 1; dz++)
-#pragma unroll
-      for (int dy = -1; dy <= 1; dy++) {
-        const int nbi = (dx + 1) * 9 + (dz + 1) * 3 + (dy + 1);
-        const T *uq = in + 3 * (int64_t)((dx * wz + dz) * wy + dy);  // neighbour q of the node
-        T s0, s1, s2;
-        if (nbi > 13) {
-          const T *s = Sh + (int64_t)((nbi - 14) * 9 + 3 * r) * nint;
-          s0 = ld_at(s, sb);
-          s1 = ld_at(s + nint, sb);
-          s2 = ld_at(s + 2 * (int64_t)nint, sb);
-        } else if (nbi == 13) {  // symmetric centre: C[r][c] = C[min][max]
-          const T *c = Sh + (int64_t)117 * nint;
-          s0 = ld_at(c + (int64_t)half_centre_index(0, r) * nint, sb);
-          s1 = ld_at(c + (int64_t)half_centre_index(r < 1 ? r : 1, r < 1 ? 1 : r) * nint, sb);
-          s2 = ld_at(c + (int64_t)half_centre_index(r, 2) * nint, sb);
-        } else {
-          const bool inside = i + dx >= 0 && i + dx < nx && j + dy >= 0 && j + dy < ny &&
-                              k + dz >= 0 && k + dz < nz;
-          s0 = s1 = s2 = T(0);
-          if (inside) {  // column r of the block neighbour n + dn stores towards n
-            const T *s = Sh + (int64_t)((12 - nbi) * 9 + r) * nint + ((dx * nz + dz) * ny + dy);
-            s0 = ld_at(s, sb);
-            s1 = ld_at(s + 3 * (int64_t)nint, sb);
-            s2 = ld_at(s + 6 * (int64_t)nint, sb);
-          }
-        }
-        acc += s0 * ld_at(uq, ub) + s1 * ld_at(uq + 1, ub) + s2 * ld_at(uq + 2, ub);
-      }
+  if (ok) {  // (w is uniform in the wave)
+    if (w == 0)
+      acc = half_row_part<T, -1>(Sh, in, nint, nx, ny, nz, wy, wz, i, j, k, r, sb, ub);
+    else if (w == 1)
+      acc = half_row_part<T, 0>(Sh, in, nint, nx, ny, nz, wy, wz, i, j, k, r, sb, ub);
+    else
+      acc = half_row_part<T, 1>(Sh, in, nint, nx, ny, nz, wy, wz, i, j, k, r, sb, ub);
+  }
+  if (w != 1) part[w >> 1][lane] = acc;
+  __syncthreads();
+  if (w != 1 || !ok) return;
+  acc = (part[0][lane] + acc) + part[1][lane];
   const uint32_t o = 3 * node + r;
   const T bb = MODE == kApply ? T(0) : b[o];
   const T dd = MODE == kJacobi ? invD[o] : T(0);
@@ -501,8 +530,8 @@ void launch_stencil_half(const DevGrid &g, int l, int mode, const T *Sh, const T
     set_error(TOP3D_ERR_ARG, "level %d too large for 32-bit stencil offsets", l);
     return;
   }
-  const uint32_t nblk = (uint32_t)((nint + 255) / 256), per = (3 * nblk + 7) / 8;
-  dim3 bs(256), gs(8 * per);
+  const uint32_t nblk = (uint32_t)((nint + 63) / 64), per = (3 * nblk + 7) / 8;
+  dim3 bs(192), gs(8 * per);
   if (mode == kApply)
     hipLaunchKernelGGL((k_stencil_half<T, kApply>), gs, bs, 0, stream(), L.nx, L.ny, L.nz,
                        L.wy, L.wz, Sh, in, out, b, invD, omega, nblk, per);
