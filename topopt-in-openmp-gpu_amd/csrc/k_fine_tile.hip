@@ -329,12 +329,45 @@ __device__ __forceinline__ void element_core_l1(const K1Vals<T> &kv, const T (&E
   l1_rows<T>(g, kv, Eh, H, std::make_integer_sequence<int, 24>{});
 }
 
+// A register copy made at this point of the program (an empty asm with a
+// "+v" operand would keep the value in its source register, whose live range
+// then overlaps the next step's load of the same variable).
+__device__ __forceinline__ float copy_here(float x) {
+  float y;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(y) : "v"(x));
+  return y;
+}
+__device__ __forceinline__ double copy_here(double x) {
+  double y;
+  asm volatile("v_mov_b64 %0, %1" : "=v"(y) : "v"(x));
+  return y;
+}
+__device__ __forceinline__ int copy_here(int x) {
+  int y;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(y) : "v"(x));
+  return y;
+}
+
+// Workgroup barrier for the LDS exchange of the tile sweeps: the wave's LDS
+// accesses complete (lgkmcnt 0; vmcnt and expcnt left at their maxima), then
+// s_barrier; the empty asm statements keep the compiler from moving memory
+// accesses across it.  __syncthreads() also releases the wave's global
+// stores to the workgroup, which waits for every vector memory operation in
+// flight, the loads of the next node plane included.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_s_waitcnt(0xC07F);
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
 struct TileArgs {
   int nelx, nely, nelz;  // elements of the level
   int wy, wz;            // node strides of the level
   int few, fez;          // fine element strides
   int TY, TZ, NX;        // tile: element columns in y and z, node planes in x
   int nty;               // tiles along y
+  uint32_t out_bytes;    // size of the output vector (below 2^32 - 32)
 };
 
 // p[idx] with the byte offset formed in 32 bits, so that the load or store
@@ -358,11 +391,34 @@ __device__ __forceinline__ T recip(T x) {
   else return T(1) / x;
 }
 
+// Stores the 3 components of a node through a buffer resource of the output
+// vector: an offset at or beyond its size (the resource's record count)
+// writes nothing, which the hardware guarantees for out-of-range buffer
+// stores, so that every lane can issue the store without a branch.
+__device__ __forceinline__ void store_node(__amdgpu_buffer_rsrc_t r, uint32_t off,
+                                           const float (&o)[3]) {
+  typedef uint32_t u3 __attribute__((ext_vector_type(3)));
+  const u3 v = {__float_as_uint(o[0]), __float_as_uint(o[1]), __float_as_uint(o[2])};
+  __builtin_amdgcn_raw_buffer_store_b96(v, r, (int)off, 0, 0);
+}
+__device__ __forceinline__ void store_node(__amdgpu_buffer_rsrc_t r, uint32_t off,
+                                           const double (&o)[3]) {
+  typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+  typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+  const uint64_t b0 = __double_as_longlong(o[0]), b1 = __double_as_longlong(o[1]),
+                 b2 = __double_as_longlong(o[2]);
+  const u4 lo = {(uint32_t)b0, (uint32_t)(b0 >> 32), (uint32_t)b1, (uint32_t)(b1 >> 32)};
+  const u2 hi = {(uint32_t)b2, (uint32_t)(b2 >> 32)};
+  __builtin_amdgcn_raw_buffer_store_b128(lo, r, (int)off, 0, 0);
+  __builtin_amdgcn_raw_buffer_store_b64(hi, r, (int)(off + 16u), 0, 0);
+}
+
 // LVL 0: fine level, one modulus per element; LVL 1: level 1, eight fine
 // moduli per element.  MODE: apply / Jacobi sweep / residual.
 // fixplane[i] != 0 if a node of plane i has a fixed DOF; planes without one
-// skip the per-node mask.
-template <typename T, int LVL, int MODE, bool BCSYM>
+// skip the per-node mask.  PIPE: the loads of a step are issued a step ahead
+// (below); else at the step's head, where the step waits for them.
+template <typename T, int LVL, int MODE, bool BCSYM, bool PIPE>
 __device__ __forceinline__ void tile_sweep_body(
     TileArgs a, std::conditional_t<LVL == 0, Blocks<T>, K1Vals<T>> B, const T *__restrict__ E,
     const T *__restrict__ in, T *__restrict__ out, const T *__restrict__ b,
@@ -410,7 +466,6 @@ __device__ __forceinline__ void tile_sweep_body(
 
   // raw values of the face of node plane i (zero outside the grid)
   auto fetch_face = [&](int i, T (&v)[2][2][3]) {
-    const bool pf = BCSYM && fixplane[i] != 0;
 #pragma unroll
     for (int py = 0; py < 2; py++)
 #pragma unroll
@@ -418,17 +473,26 @@ __device__ __forceinline__ void tile_sweep_body(
         // (the march only visits planes i <= nelx + 2)
         const uint32_t nd = (uint32_t)i * sx + n00c + (elem_ok ? noff[py][pz] : 0u);
         const T *pn = at32(in, 3u * nd);
-        T v0 = pn[0], v1 = pn[1], v2 = pn[2];
-        if (BCSYM && pf) {
-          const uint8_t m = fix[nd];
-          if (m & 1) v0 = 0;
-          if (m & 2) v1 = 0;
-          if (m & 4) v2 = 0;
-        }
-        v[py][pz][0] = v0;
-        v[py][pz][1] = v1;
-        v[py][pz][2] = v2;
+        v[py][pz][0] = pn[0];
+        v[py][pz][1] = pn[1];
+        v[py][pz][2] = pn[2];
       }
+  };
+  // BCSYM: the fixed DOFs of the face of plane i read as zero (applied when
+  // the face is consumed, on the planes that hold fixed DOFs)
+  auto mask_face = [&](int i, T (&v)[2][2][3]) {
+    if (BCSYM && fixplane[i] != 0) {
+#pragma unroll
+      for (int py = 0; py < 2; py++)
+#pragma unroll
+        for (int pz = 0; pz < 2; pz++) {
+          const uint32_t nd = (uint32_t)i * sx + n00c + (elem_ok ? noff[py][pz] : 0u);
+          const uint8_t m = fix[nd];
+          if (m & 1) v[py][pz][0] = 0;
+          if (m & 2) v[py][pz][1] = 0;
+          if (m & 4) v[py][pz][2] = 0;
+        }
+    }
   };
   T vq[2][2][3];
 
@@ -438,40 +502,91 @@ __device__ __forceinline__ void tile_sweep_body(
 #pragma unroll
     for (int c = 0; c < 3; c++) pendT[f][c] = 0;
 
-  // element slab i (node planes i, i+1): Fc the transformed face of plane
-  // i, Fn that of plane i+1 (loaded here)
-  auto step = [&](int i, const T (&Fc)[4][3], T (&Fn)[4][3]) {
-    // the owned node's right-hand side, inverse diagonal and mask of plane
-    // i, loaded ahead of the element work that hides their latency
-    const bool emit = i >= i0;  // uniform
-    const uint32_t nd = (uint32_t)i * sx + nown;
-    const uint32_t d0 = 3u * nd;
-    // read only by owners (own_ok) of an emitted plane
-    T bb[3], di[3], u0[3];
+  // PIPE: every load is issued a step ahead of its use, so that its latency
+  // hides behind a step of element work; per step, in issue order:
+  //   head: wait for the raw face of plane i+1 and the slab's modulus (issued
+  //         at the last step's head), then issue those of the next step;
+  //   end:  the owned node's output of plane i, then the loads of the owned
+  //         node's operands for step i+1 (b, invD, mask), issued after this
+  //         step's operands have been used so that one set is live at a time.
+  // Every path through a step issues the same memory operations, so the
+  // waits leave the younger ones in flight (the memory counter is in order).
+  // This pays where the element work per load is large (FP64, level 1); the
+  // FP32 fine level, whose waves are bound by their VALU issue, runs faster
+  // without the copies and the scheduling barrier it takes.
+  struct NodeIn {  // the owned node's b, invD and fixed-DOF mask
+    T bb[3], di[3];
     uint8_t m;
-    if (own_ok) {  // (on the first, unemitted slab the loads are unused)
-      if (MODE != kModeApply) {
-        const T *pb = at32(b, d0);
-        bb[0] = pb[0];
-        bb[1] = pb[1];
-        bb[2] = pb[2];
-      }
-      if (MODE == kModeJacobi) {
-        const T *pd = at32(invD, d0);
-        di[0] = pd[0];
-        di[1] = pd[1];
-        di[2] = pd[2];
-      }
-      // the owned node's input: its value for the update and the
-      // identity rows (unmasked even when BCSYM masks the face loads)
-      const T *pn = at32(in, d0);
-      u0[0] = pn[0];
-      u0[1] = pn[1];
-      u0[2] = pn[2];
-      m = fixplane[i] != 0 ? fix[nd] : uint8_t(0);
+  };
+  auto load_node = [&](int i, NodeIn &sv) {
+    // threads that own no node read node 0 of the plane (in bounds) and
+    // never use it
+    const uint32_t nd = (uint32_t)i * sx + (own_ok ? nown : 0u);
+    const uint32_t d0 = 3u * nd;
+    if (MODE != kModeApply) {
+      const T *pb = at32(b, d0);
+      sv.bb[0] = pb[0];
+      sv.bb[1] = pb[1];
+      sv.bb[2] = pb[2];
     }
-    fetch_face(i + 1, vq);
-    face_wht<T>(vq, Fn);
+    if (MODE == kModeJacobi) {
+      const T *pd = at32(invD, d0);
+      sv.di[0] = pd[0];
+      sv.di[1] = pd[1];
+      sv.di[2] = pd[2];
+    }
+    sv.m = fixplane[i] != 0 ? fix[nd] : uint8_t(0);
+  };
+  // (the march visits slabs i <= nelx + 1)
+  auto load_modulus = [&](int i) -> T {
+    if constexpr (LVL == 0) return *at32(E, (uint32_t)i * a.fez * a.few + e00c);
+    return T(0);
+  };
+  // the output vector as a buffer resource (stride 0, size in bytes); the
+  // offset out_bytes addresses nothing
+  const __amdgpu_buffer_rsrc_t orsrc =
+      __builtin_amdgcn_make_buffer_rsrc(out, (short)0, (int)a.out_bytes, 0x00020000);
+  NodeIn nv{};
+  T Ecur = 0;
+  // u0: the owned node (j+1, k+1) of plane i, node (1, 1) of this thread's
+  // face of plane i; u1: that of plane i+1 (unmasked even when BCSYM masks
+  // the faces)
+  T u0[3] = {0, 0, 0}, u1[3] = {0, 0, 0};
+
+  // element slab i (node planes i, i+1): Fc the transformed face of plane
+  // i, Fn that of plane i+1 (from the raw face in vq)
+  auto step = [&](int i, const T (&Fc)[4][3], T (&Fn)[4][3]) {
+    const bool emit = i >= i0;  // uniform
+    const uint32_t d0 = 3u * ((uint32_t)i * sx + nown);
+    T Ee = 0;
+    if constexpr (PIPE) {
+      // The values of the face and the modulus are copied into registers of
+      // their own here, where they have been waited for: the loads issued
+      // below then land in the registers they leave (an empty asm with a
+      // "+v" operand fixes the transformed face here).
+#pragma unroll
+      for (int c = 0; c < 3; c++) u1[c] = copy_here(vq[1][1][c]);
+      mask_face(i + 1, vq);
+      face_wht<T>(vq, Fn);
+#pragma unroll
+      for (int f = 0; f < 4; f++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) asm volatile("" : "+v"(Fn[f][c]));
+      if constexpr (LVL == 0) Ee = copy_here(Ecur);
+      // nothing moves across
+      __builtin_amdgcn_sched_barrier(0);
+      // (unconditional, the last step repeats its plane and slab)
+      Ecur = load_modulus(min(i + 1, i1));
+      fetch_face(min(i + 2, i1 + 1), vq);
+    } else {
+      fetch_face(i + 1, vq);
+      Ee = load_modulus(i);
+      load_node(i, nv);
+#pragma unroll
+      for (int c = 0; c < 3; c++) u1[c] = vq[1][1][c];
+      mask_face(i + 1, vq);
+      face_wht<T>(vq, Fn);
+    }
     T h[8][3];
 #pragma unroll
     for (int f = 0; f < 4; f++)
@@ -483,10 +598,7 @@ __device__ __forceinline__ void tile_sweep_body(
     // g[0][c] (the element's rigid translations) is zero at both levels and
     // is never formed or read below
     T g[8][3];
-    T Ee = 0;
     if constexpr (LVL == 0) {
-      // (the march only visits slabs i <= nelx + 1)
-      Ee = *at32(E, (uint32_t)i * a.fez * a.few + e00c);
       element_core<T>(B, Ee, h, g);
       if constexpr (kDiag) {
         if (i < i0 && own_ok) {  // first slab: the neighbours' moduli from memory
@@ -525,6 +637,9 @@ __device__ __forceinline__ void tile_sweep_body(
       }
       element_core_l1<T>(B, Eh, h, g);
     }
+    T o[3] = {0, 0, 0};
+    const T *bb = nv.bb, *di = nv.di;
+    const uint8_t m = nv.m;
     if (emit) {
       // plane i: x- face of slab i plus the pending x+ face of slab i-1,
       // summed in the transformed basis
@@ -536,34 +651,38 @@ __device__ __forceinline__ void tile_sweep_body(
 #pragma unroll
         for (int c = 0; c < 3; c++) totT[f][c] = (g[2 * f][c] + g[2 * f + 1][c]) + pendT[f][c];
       face_iwht<T>(totT, tot);
-      T *buf = xch + (size_t)(i & 1) * nthr * XS;
+      // this thread's slot, formed anew each step (the addresses of its
+      // neighbours' slots are then a few adds here rather than registers
+      // held across the march)
+      const int ts = copy_here(t);
+      T *buf = xch + (size_t)(i & 1) * nthr * XS + XS * ts;
       if (active) {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-          buf[XS * t + c] = tot[0][1][c];      // node (j, k+1)  -> thread (ty-1, tz)
-          buf[XS * t + 3 + c] = tot[1][0][c];  // node (j+1, k)  -> thread (ty, tz-1)
-          buf[XS * t + 6 + c] = tot[0][0][c];  // node (j, k)    -> thread (ty-1, tz-1)
+          buf[c] = tot[0][1][c];            // node (j, k+1)  -> thread (ty-1, tz)
+          buf[3 + c] = tot[1][0][c];           // node (j+1, k)  -> thread (ty, tz-1)
+          buf[6 + c] = tot[0][0][c];           // node (j, k)    -> thread (ty-1, tz-1)
         }
-        if constexpr (kDiag) buf[XS * t + 9] = Ee;
+        if constexpr (kDiag) buf[9] = Ee;
       }
-      __syncthreads();
+      lds_barrier();
       if (own_ok) {
         T kin[3] = {tot[1][1][0], tot[1][1][1], tot[1][1][2]};
         T es = Ee;  // kDiag: moduli of this slab's 4 elements at the node
         if (!last_y) {
 #pragma unroll
-          for (int c = 0; c < 3; c++) kin[c] += buf[XS * (t + 1) + c];
-          if constexpr (kDiag) es += buf[XS * (t + 1) + 9];
+          for (int c = 0; c < 3; c++) kin[c] += buf[XS + c];
+          if constexpr (kDiag) es += buf[XS + 9];
         }
         if (!last_z) {
 #pragma unroll
-          for (int c = 0; c < 3; c++) kin[c] += buf[XS * (t + a.TY) + 3 + c];
-          if constexpr (kDiag) es += buf[XS * (t + a.TY) + 9];
+          for (int c = 0; c < 3; c++) kin[c] += buf[XS * a.TY + 3 + c];
+          if constexpr (kDiag) es += buf[XS * a.TY + 9];
         }
         if (!last_y && !last_z) {
 #pragma unroll
-          for (int c = 0; c < 3; c++) kin[c] += buf[XS * (t + a.TY + 1) + 6 + c];
-          if constexpr (kDiag) es += buf[XS * (t + a.TY + 1) + 9];
+          for (int c = 0; c < 3; c++) kin[c] += buf[XS * (a.TY + 1) + 6 + c];
+          if constexpr (kDiag) es += buf[XS * (a.TY + 1) + 9];
         }
         T dd[3] = {di[0], di[1], di[2]};
         if constexpr (kDiag) {
@@ -573,7 +692,6 @@ __device__ __forceinline__ void tile_sweep_body(
 #pragma unroll
           for (int c = 0; c < 3; c++) dd[c] = ((m >> c) & 1) ? T(1) : rs * B.ikd[c];
         }
-        T o[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
           const T uc = u0[c];
@@ -589,12 +707,13 @@ __device__ __forceinline__ void tile_sweep_body(
             o[c] = bb[c] - kv;
           }
         }
-        T *po = at32(out, d0);
-        po[0] = o[0];
-        po[1] = o[1];
-        po[2] = o[2];
       }
     }
+    // every lane stores (the step's last memory operation, so that the next
+    // step's wait for its face leaves it in flight): the owned node of an
+    // emitted plane, else nothing
+    store_node(orsrc, emit && own_ok ? d0 * (uint32_t)sizeof(T) : a.out_bytes, o);
+    if constexpr (PIPE) load_node(min(i + 1, i1), nv);
 #pragma unroll
     for (int c = 0; c < 3; c++) pendT[0][c] = -g[1][c];
 #pragma unroll
@@ -606,13 +725,27 @@ __device__ __forceinline__ void tile_sweep_body(
   // the march; the transformed face of plane i+1 becomes that of plane i
   T Fa[4][3], Fb[4][3];
   fetch_face(i0 - 1, vq);
+#pragma unroll
+  for (int c = 0; c < 3; c++) u0[c] = vq[1][1][c];
+  mask_face(i0 - 1, vq);
   face_wht<T>(vq, Fa);
+  if constexpr (PIPE) {
+    // the loads a step's head and end would have issued before the first
+    // step (the store writes nothing; the first step emits no plane and
+    // leaves the node operands unused)
+    Ecur = load_modulus(i0 - 1);
+    fetch_face(i0, vq);
+    store_node(orsrc, a.out_bytes, u0);
+    load_node(i0 - 1, nv);
+  }
   for (int i = i0 - 1; i <= i1; i++) {
     step(i, Fa, Fb);
 #pragma unroll
     for (int f = 0; f < 4; f++)
 #pragma unroll
       for (int c = 0; c < 3; c++) Fa[f][c] = Fb[f][c];
+#pragma unroll
+    for (int c = 0; c < 3; c++) u0[c] = u1[c];
   }
   if (MODE != kModeResidual && dotp) {  // one partial per workgroup, fixed order
 #pragma unroll
@@ -629,13 +762,18 @@ __device__ __forceinline__ void tile_sweep_body(
   }
 }
 
+// Waves per SIMD the register allocation must allow: FP64 fine level 3
+// (three 4-wave workgroups of 17 x 12 columns per CU, at most 168 VGPRs);
+// otherwise what the launch bounds already imply (one workgroup per CU).
 template <typename T, int LVL, int MODE, bool BCSYM>
-__global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_tile_sweep(
+__global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : LVL == 0 ? 3 : 2))) void k_tile_sweep(
     TileArgs a, std::conditional_t<LVL == 0, Blocks<T>, K1Vals<T>> B, const T *__restrict__ E,
     const T *__restrict__ in, T *__restrict__ out, const T *__restrict__ b,
     const T *__restrict__ invD, T omega, const uint8_t *__restrict__ fix,
     const int *__restrict__ fixplane, double *__restrict__ dotp) {
-  tile_sweep_body<T, LVL, MODE, BCSYM>(a, B, E, in, out, b, invD, omega, fix, fixplane, dotp);
+  tile_sweep_body<T, LVL, MODE, BCSYM, true>(a, B, E, in, out, b, invD, omega, fix, fixplane,
+                                             dotp);
 }
 
 // fine level in FP32: workgroups of up to 1024 threads; two 9-wave
@@ -646,7 +784,8 @@ __global__ __launch_bounds__(1024) void k_tile_sweep_f32(
     float *__restrict__ out, const float *__restrict__ b, const float *__restrict__ invD,
     float omega, const uint8_t *__restrict__ fix, const int *__restrict__ fixplane,
     double *__restrict__ dotp) {
-  tile_sweep_body<float, 0, MODE, BCSYM>(a, B, E, in, out, b, invD, omega, fix, fixplane, dotp);
+  tile_sweep_body<float, 0, MODE, BCSYM, false>(a, B, E, in, out, b, invD, omega, fix, fixplane,
+                                                dotp);
 }
 
 // ---------------------------------------------------------------------------
@@ -834,7 +973,9 @@ static int launch_tile(const DevGrid &g, int mode, const T *E, const T *in, T *o
   }
   const Level &L = g.lv[LVL];
   const int64_t nmod = LVL == 0 ? g.nelem : level1_moduli_count(g);
-  if ((int64_t)sizeof(T) * 3 * L.nodes() >= (int64_t(1) << 32) ||
+  // (the output's byte size also bounds the store offset that writes
+  // nothing, plus 16, in 32 bits)
+  if ((int64_t)sizeof(T) * 3 * L.nodes() >= (int64_t(1) << 32) - 32 ||
       (int64_t)sizeof(T) * nmod >= (int64_t(1) << 32)) {
     set_error(TOP3D_ERR_ARG, "grid too large for 32-bit offsets (%lld nodes)",
               (long long)L.nodes());
@@ -882,6 +1023,7 @@ static int launch_tile(const DevGrid &g, int mode, const T *E, const T *in, T *o
   a.TZ = s.TZ;
   a.NX = s.NX;
   a.nty = s.nty;
+  a.out_bytes = (uint32_t)(sizeof(T) * 3 * L.nodes());
   dim3 grid(s.ntx, s.nty * s.ntz);
   const int nwg = s.ntx * s.nty * s.ntz;
   if (dotp && nwg > dot_cap) dotp = nullptr;
