@@ -874,9 +874,12 @@ struct TileShape {
 // workgroup per CU); for the FP64 fine level 17 x 12 (4 waves) ran the
 // outer CG's apply in 73.6 us (repeated on one buffer) against 75.8 us
 // for 65 x 7, 81-112 us for 33 x 7/12/15, 25 x 20, 45 x 11 and 17 x 24;
-// wide, short x-marches for level 1:
-// in FP32, 65 x 11 (12 waves, 3 per SIMD, 231 workgroups) ran 17.6 us against
-// 18.6 us for 65 x 12 (13 waves, 198 workgroups).  The x-march length
+// for level 1 (128x64x64) 65 x 7 with the x-march length below ran 15.5 us
+// in FP32 and 23.6 us in FP64 (repeated on one buffer set; 20.6 and 33.7 us
+// with the memory-side cache flushed) against 15.8 / 27.1 us for 65 x 11
+// (NX 4, FP32) and 65 x 6 (NX 8, FP64), and 15.8-19.3 / 24.3-30.5 us for
+// 65 x 3/5/6/11, 33 x 6/9/11/12/15/21 and 17 x 12/23.
+// The x-march length
 // follows from the number of workgroups the device holds at once (`slots`):
 // a launch that overflows them by a few
 // workgroups pays a whole extra round, so NX is chosen to minimise
@@ -892,7 +895,7 @@ static TileShape tile_shape(const Level &L, int lvl, int elem_bytes, int &nx_fix
     int ycap, tz, nx;
   };
   static constexpr Rule kRules[2][2] = {{{17, 12, 0}, {33, 17, 0}},  // level 0: f64, f32
-                                        {{65, 6, 8}, {65, 11, 4}}};  // level 1
+                                        {{65, 7, 0}, {65, 7, 0}}};   // level 1
   const Rule r = kRules[lvl == 0 ? 0 : 1][elem_bytes == 4 ? 1 : 0];
   s.TY = std::max(2, std::min(L.nely + 1, r.ycap));
   s.TZ = std::max(2, std::min({L.nelz + 1, r.tz, max_threads / s.TY}));
