@@ -15,7 +15,10 @@ import top3d_amd
 
 pytestmark = pytest.mark.gpu
 
-SIZES = [(16, 8, 8, 3), (24, 16, 8, 4)]
+# (the last size spans several tiles in y and z at levels 0 and 1: 2 x 2 of
+# the FP32 fine kernel's 33 x 17, 3 x 3 of the FP64 one's 17 x 12, 1 x 3 of
+# level 1's 65 x 7 columns)
+SIZES = [(16, 8, 8, 3), (24, 16, 8, 4), (40, 36, 28, 3)]
 
 
 def relerr(got, ref):

@@ -159,7 +159,7 @@ def test_autograd_compliance_gradient_fd():
         xp_[e] += h
         xm_[e] -= h
         fd = (float(compliance(xp_)) - float(compliance(xm_))) / (2 * h)
-        assert float(grad_x[e]) == pytest.approx(fd, rel=1e-6, abs=1e-7 * abs(float(c)))
+        assert float(grad_x[e]) == pytest.approx(fd, rel=1e-6, abs=1e-7 * abs(c.item()))
     g.close()
 
 
