@@ -266,8 +266,8 @@ def test_autograd_c3_gradient_fd_and_oracle_sensitivity():
     g.solve(xphys, F, U, tol=1e-12)
     c_o, dc_o = o.sensitivity(xphys.cpu().numpy(), U.cpu().numpy())
     idx = g.interior_elements()
-    assert abs(float(c) - c_o) <= 1e-10 * abs(c_o)
-    assert abs(float(c) - float(torch.dot(F, U))) <= 1e-10 * abs(c_o)
+    assert abs(c.item() - c_o) <= 1e-10 * abs(c_o)
+    assert abs(c.item() - float(torch.dot(F, U))) <= 1e-10 * abs(c_o)
     assert relerr(xp.grad.cpu().numpy()[idx], dc_o[idx]) <= 1e-10
     grad_x = g.filter_grad(rmin, xp.grad).cpu().numpy()
 
